@@ -1,0 +1,219 @@
+"""Headline benchmark: batched lvl1 programmable bootstrapping on one MI355X.
+
+Workload (one "step"): a batch of B = 8192 lvl0 LWE ciphertexts (n0 = 636,
+uniform binary key, alpha0 = 2^-15) holding Z_8 plaintexts with one padding
+bit, bootstrapped to lvl1 (N = 1024, k = 1, l = 3, Bgbit = 6) through a
+random 8-entry lookup table: one fhe_pbs_lvl01_dev launch = BlindRotate +
+SampleExtract for all B ciphertexts (TFHEpp GateBootstrappingTLWE2TLWEFFT).
+Inputs, key and outputs stay resident in HBM during the timed region; the
+time is taken with HIP events on the stream the kernel runs on.
+
+Prints one JSON line (see the repository's measurement contract): PBS/s,
+the roofline fraction of the dominant kernel, and a CPU baseline timed on
+this host's cores.
+"""
+
+import argparse
+import concurrent.futures as cf
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "lvl1 PBS/sec @batch 8192 (MFMA-split vs FP64); ms per 2^14-row encrypted query"
+PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+N, L, BGBIT = 1024, 3, 6
+
+
+def flops_per_pbs(n0):
+    """FFT-based algorithmic flops of one lvl1 PBS (k = 1):
+    n0 * [((k+1)l + (k+1)) * 5 (N/2) log2(N/2) + (k+1)l (k+1) * 8 (N/2)]."""
+    M = N // 2
+    return n0 * ((2 * L + 2) * 5 * M * 9 + 2 * L * 2 * 8 * M)
+
+
+def z8_testvector(lut):
+    j = np.arange(N)
+    box = (j + 64) >> 7
+    vals = np.array(lut, np.int64)[box % 8] << 28
+    vals = np.where(box == 8, -vals, vals)
+    return (vals % (1 << 32)).astype(np.uint32)
+
+
+def load_traffic(variant, preset, batch):
+    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary)."""
+    path = os.path.join(ROOT, "profiles", f"pbs_lvl01_{variant}_{preset}_b{batch}.traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(threads, seconds_budget=12.0):
+    """Reference TFHEpp (oracle/_ref, compiled from the reference sources) PBS
+    throughput on this host, falling back to the oracle's C restatement."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    import cufhedb_amd as F
+
+    ref = O.ref()
+    if ref is not None:
+        p = F.preset(F.PRESET_REF672)
+        kind, fn_name = "reference", "ref_pbs_lvl01"
+    else:
+        p = F.preset(F.PRESET_CFG636)
+        kind, fn_name = "port", "orc_pbs_u32"
+    sk = F.SecretKeys(p, 99)
+    bkfft = O.bk_to_fft32(F.keygen_bk_lvl01(sk, 98))
+    rng = np.random.default_rng(5)
+    tv = z8_testvector(rng.integers(0, 8, 8))
+    nsamp = 4 * threads
+    c0 = F.tlwe_encrypt_u32(97, 0, sk.s0, (rng.integers(0, 8, nsamp) << 28).astype(np.uint32),
+                            p.alpha0)
+    n0 = p.n0
+
+    def one(i):
+        out = np.zeros(N + 1, np.uint32)
+        row = np.ascontiguousarray(c0[i % nsamp])
+        if kind == "reference":
+            ref.ref_pbs_lvl01(O.P(out), O.P(row), O.P(bkfft), O.P(tv))
+        else:
+            O.orc().orc_pbs_u32(O.P(out), O.P(row), n0, O.P(bkfft), O.P(tv), N, 10, L, BGBIT)
+        return 1
+
+    # calibrate on one PBS, then size the sample to ~seconds_budget of CPU work
+    t = time.perf_counter()
+    one(0)
+    per = time.perf_counter() - t
+    total = max(threads, int(seconds_budget / max(per, 1e-4)))
+    total = (total + threads - 1) // threads * threads
+    with cf.ThreadPoolExecutor(threads) as ex:
+        t = time.perf_counter()
+        done = sum(ex.map(one, range(total)))
+        wall = time.perf_counter() - t
+    impl = ("TFHEpp GateBootstrappingTLWE2TLWEFFT<lvl01param> (spqlios FP64 FFT) compiled from "
+            "the reference sources, its fixed n0=672" if kind == "reference" else
+            "oracle C restatement, n0=636")
+    return {
+        "value": done / wall,
+        "unit": "PBS/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": f"{done} lvl1 PBS ({impl}), one ciphertext per thread at a time, "
+                  f"{wall:.2f} s wall on {threads} host threads",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--preset", choices=["cfg636", "ref672"], default="cfg636")
+    ap.add_argument("--variant", choices=["fp64"], default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by ciphertext with replicated keys, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import cufhedb_amd as F
+
+    which = F.PRESET_CFG636 if args.preset == "cfg636" else F.PRESET_REF672
+    p = F.preset(which)
+    sk = F.SecretKeys(p, 20261202)
+    bk = F.keygen_bk_lvl01(sk, 20261203)
+    ctx = F.Context(p, 0)
+    ctx.load_bk(1, bk, F.BK_INT)
+    ctx.set_pbs_variant({"fp64": F.PBS_FP64}[args.variant])
+
+    B = args.batch
+    rng = np.random.default_rng(11)
+    m = rng.integers(0, 8, B)
+    lut = rng.integers(0, 8, 8)
+    tv = z8_testvector(lut)
+    c0 = F.tlwe_encrypt_u32(20261204, 0, sk.s0, (m << 28).astype(np.uint32), p.alpha0)
+    d_in = F.DeviceBuffer.from_array(c0)
+    d_tv = F.DeviceBuffer.from_array(tv)
+    d_out = F.DeviceBuffer(B * (N + 1) * 4)
+    L_ = F.lib()
+
+    def step():
+        F.check(L_.fhe_pbs_lvl01_dev(ctx.h, d_in.ptr, d_tv.ptr, 0, d_out.ptr, B, None))
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    print(f"bench: warmup done ({args.warmup} x {B} PBS)", file=sys.stderr, flush=True)
+    F.check(L_.fhe_timer_start(ctx.h, None))
+    for _ in range(args.steps):
+        step()
+    ms = __import__("ctypes").c_float()
+    F.check(L_.fhe_timer_stop(ctx.h, None, __import__("ctypes").byref(ms)))
+    total_ms = float(ms.value)
+    ms_per_step = total_ms / args.steps
+    value = B * args.steps / (total_ms / 1e3)
+
+    out = d_out.to_array(np.uint32, (B, N + 1))
+    ph = F.tlwe_phase_u32(out, sk.s1)
+    dec = np.round(ph.view(np.int32).astype(np.int64) / (1 << 28)).astype(np.int64) % 16
+    failures = int(np.sum(dec != lut[m]))
+    print(f"bench: timed {args.steps} steps, {ms_per_step:.3f} ms/step, "
+          f"{failures} decryption failures", file=sys.stderr, flush=True)
+
+    fl = flops_per_pbs(p.n0)
+    achieved = fl * B / (ms_per_step / 1e3) / 1e12
+    roofline = {
+        "bound": "mfma",
+        "achieved": achieved,
+        "peak": PEAK_FP64_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / PEAK_FP64_TFLOPS,
+        "traffic": load_traffic(args.variant, args.preset, B),
+        "kernel": "k_blind_rotate1 (FP64 VALU; MI355X FP64 vector == FP64 matrix peak)",
+        "flops_per_pbs": fl,
+    }
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "PBS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded keys and Z_8 plaintexts, random 8-entry LUT)",
+        "config": {
+            "workload": f"lvl1 PBS (BlindRotate+SampleExtract), batch {B}, n0={p.n0}, "
+                        f"N=1024 k=1 l=3 Bgbit=6, {args.variant}",
+            "batch": B,
+            "n0": int(p.n0),
+            "variant": args.variant,
+        },
+        "roofline": roofline,
+        "decrypt_failures": failures,
+        "device": F.device_name(),
+    }
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        res["cpu_baseline"] = cpu_baseline(threads)
+    else:
+        res["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,418 @@
+"""Python host interface to the MI355X TFHE bootstrapping library.
+
+This package is a thin ctypes layer over ``libcufhedb_amd.so`` (HIP kernels
+for gfx950 behind the extern "C" ABI declared in ``include/cufhedb_amd.h``).
+It mirrors the reference's operator surface for the bootstrapping hot path:
+
+* ``Context.load_bk`` / ``Context.load_ksk``   -- cufft::CuFFT_Torus::LoadBK
+  (src/cuHEDB/fft_gpu/cufft_gpu.cuh:175-184), keys resident on the device;
+* ``Context.pbs_lvl01``                         -- GateBootstrappingTLWE2TLWEFFT
+  (cufft_gpu.cuh:250-271), batched;
+* ``Context.iks``                               -- TFHEpp IdentityKeySwitch
+  (thirdparty/TFHEpp/src/keyswitch.cpp:12-48), batched on the GPU;
+* ``Context.gate`` / ``msb_gate`` / ``ide_gate``  -- cuHEDB:: operators
+  (src/cuHEDB/gatebootstrap_gpu.cuh:31-170).
+
+Arrays are numpy arrays in TFHEpp's aggregate layouts (a TLWE is n+1 torus
+words, b last).  The library must be built (``python -c "import
+__graft_entry__ as g; g.build()"`` or ``make``); importing this package
+without it raises, so no computation can silently fall back to the CPU.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcufhedb_amd.so")
+
+FHE_OK = 0
+FHE_ERR_ARG = 1
+FHE_ERR_NOT_LOADED = 2
+FHE_ERR_DEVICE = 3
+FHE_ERR_ALLOC = 4
+FHE_ERR_UNSUPPORTED = 5
+
+PRESET_REF672 = 0
+PRESET_CFG636 = 1
+BK_INT = 0
+BK_FFT = 1
+PBS_FP64 = 0
+
+GATE_NAND, GATE_AND, GATE_OR, GATE_NOR, GATE_XNOR = range(5)
+LOGIC = 1
+ARITHMETIC = 0
+
+LVL1_N = 1024
+LVL2_N = 2048
+LVL1_MU = 1 << 29
+
+
+class FheError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fhe status {code}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("n0", ctypes.c_uint32),
+        ("key0_hw", ctypes.c_uint32),
+        ("key1_hw", ctypes.c_uint32),
+        ("key2_hw", ctypes.c_uint32),
+        ("alpha0", ctypes.c_double),
+        ("alpha1", ctypes.c_double),
+        ("alpha2", ctypes.c_double),
+    ]
+
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_d = ctypes.c_double
+_sz = ctypes.c_size_t
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "fhe_params_preset": (_i, [_i, _vp]),
+    "fhe_last_error": (ctypes.c_char_p, []),
+    "fhe_keygen_secret": (_i, [_vp, _u64, _vp, _vp, _vp]),
+    "fhe_keygen_bk_lvl01": (_i, [_vp, _u64, _vp, _vp, _vp]),
+    "fhe_keygen_bk_lvl02": (_i, [_vp, _u64, _vp, _vp, _vp]),
+    "fhe_keygen_ksk": (_i, [_vp, _i, _u64, _vp, _vp, _vp]),
+    "fhe_ksk_words": (_sz, [_vp, _i]),
+    "fhe_tlwe_encrypt_u32": (_i, [_u64, _u64, _vp, _u32, _vp, _d, _vp, _i]),
+    "fhe_tlwe_encrypt_u64": (_i, [_u64, _u64, _vp, _u32, _vp, _d, _vp, _i]),
+    "fhe_trlwe_encrypt_u32": (_i, [_u64, _u64, _vp, _vp, _d, _vp, _i]),
+    "fhe_trlwe_encrypt_u64": (_i, [_u64, _u64, _vp, _vp, _d, _vp, _i]),
+    "fhe_trgsw_encrypt_u32": (_i, [_u64, _u64, _vp, ctypes.c_int32, _d, _vp]),
+    "fhe_trgsw_encrypt_u64": (_i, [_u64, _u64, _vp, ctypes.c_int64, _d, _vp]),
+    "fhe_ctx_create": (_i, [_vp, _i, _vp]),
+    "fhe_ctx_destroy": (_i, [_vp]),
+    "fhe_ctx_sync": (_i, [_vp]),
+    "fhe_ctx_stream": (_vp, [_vp]),
+    "fhe_load_bk": (_i, [_vp, _i, _vp, _i]),
+    "fhe_load_ksk": (_i, [_vp, _i, _vp]),
+    "fhe_set_pbs_variant": (_i, [_vp, _i]),
+    "fhe_pbs_lvl01_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp]),
+    "fhe_blind_rotate_lvl01_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp]),
+    "fhe_pbs_lvl02_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp]),
+    "fhe_sample_extract_lvl1_dev": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
+    "fhe_sample_extract_lvl2_dev": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
+    "fhe_iks_dev": (_i, [_vp, _i, _vp, _vp, _i, _vp]),
+    "fhe_external_product_lvl1_dev": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "fhe_external_product_lvl2_dev": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "fhe_ifft_lvl1_dev": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "fhe_fft_lvl1_dev": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "fhe_ifft_lvl2_dev": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "fhe_fft_lvl2_dev": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "fhe_gate_lvl1_dev": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp]),
+    "fhe_msb_gate_lvl1_dev": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
+    "fhe_ide_gate_lvl1_dev": (_i, [_vp, _vp, _vp, _u32, _i, _vp]),
+    "fhe_pbs_lvl01": (_i, [_vp, _vp, _vp, _i, _vp, _i]),
+    "fhe_iks": (_i, [_vp, _i, _vp, _vp, _i]),
+    "fhe_malloc": (_i, [_vp, _sz]),
+    "fhe_free": (_i, [_vp]),
+    "fhe_memcpy_h2d": (_i, [_vp, _vp, _sz]),
+    "fhe_memcpy_d2h": (_i, [_vp, _vp, _sz]),
+    "fhe_memset_dev": (_i, [_vp, _i, _sz]),
+    "fhe_device_sync": (_i, []),
+    "fhe_timer_start": (_i, [_vp, _vp]),
+    "fhe_timer_stop": (_i, [_vp, _vp, _vp]),
+    "fhe_device_name": (_i, [_vp, _i]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libcufhedb_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `make -j8` or __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def check(rc):
+    if rc != FHE_OK:
+        raise FheError(rc, lib().fhe_last_error().decode(errors="replace"))
+
+
+def preset(which):
+    p = Params()
+    check(lib().fhe_params_preset(which, ctypes.byref(p)))
+    return p
+
+
+# ---------------------------------------------------------------- key material
+class SecretKeys:
+    """lvl0 / lvl1 / lvl2 secret keys (TFHEpp lweKey, key.cpp:8-17), seeded."""
+
+    def __init__(self, params, seed):
+        self.params = params
+        self.seed = seed
+        self.s0 = np.zeros(params.n0, np.uint32)
+        self.s1 = np.zeros(LVL1_N, np.uint32)
+        self.s2 = np.zeros(LVL2_N, np.uint64)
+        check(lib().fhe_keygen_secret(ctypes.byref(params), seed, _ptr(self.s0), _ptr(self.s1),
+                                      _ptr(self.s2)))
+
+
+def keygen_bk_lvl01(sk, seed):
+    """Integer BK [n0][6][2][1024] uint32 (TFHEpp BootstrappingKey<lvl01param>)."""
+    bk = np.zeros((sk.params.n0, 6, 2, LVL1_N), np.uint32)
+    check(lib().fhe_keygen_bk_lvl01(ctypes.byref(sk.params), seed, _ptr(sk.s0), _ptr(sk.s1),
+                                    _ptr(bk)))
+    return bk
+
+
+def keygen_bk_lvl02(sk, seed):
+    bk = np.zeros((sk.params.n0, 8, 2, LVL2_N), np.uint64)
+    check(lib().fhe_keygen_bk_lvl02(ctypes.byref(sk.params), seed, _ptr(sk.s0), _ptr(sk.s2),
+                                    _ptr(bk)))
+    return bk
+
+
+def ksk_shape(params, kind):
+    if kind == 10:
+        return (LVL1_N, 2, 1023, params.n0 + 1)
+    if kind == 20:
+        return (LVL2_N, 2, 1023, params.n0 + 1)
+    if kind == 21:
+        return (LVL2_N, 10, 7, LVL1_N + 1)
+    raise ValueError(kind)
+
+
+def keygen_ksk(sk, kind, seed):
+    """KeySwitchingKey [dn][t][2^basebit-1][tn+1] uint32 (TFHEpp ikskgen)."""
+    ksk = np.empty(ksk_shape(sk.params, kind), np.uint32)
+    sdom = sk.s1 if kind == 10 else sk.s2
+    stgt = sk.s1 if kind == 21 else sk.s0
+    check(lib().fhe_keygen_ksk(ctypes.byref(sk.params), kind, seed, _ptr(sdom), _ptr(stgt),
+                               _ptr(ksk)))
+    return ksk
+
+
+def tlwe_encrypt_u32(seed, index0, key, m, alpha):
+    m = np.ascontiguousarray(m, np.uint32)
+    n = key.shape[0]
+    out = np.zeros((m.shape[0], n + 1), np.uint32)
+    check(lib().fhe_tlwe_encrypt_u32(seed, index0, _ptr(key), n, _ptr(m), alpha, _ptr(out),
+                                     m.shape[0]))
+    return out
+
+
+def tlwe_encrypt_u64(seed, index0, key, m, alpha):
+    m = np.ascontiguousarray(m, np.uint64)
+    n = key.shape[0]
+    out = np.zeros((m.shape[0], n + 1), np.uint64)
+    check(lib().fhe_tlwe_encrypt_u64(seed, index0, _ptr(key), n, _ptr(m), alpha, _ptr(out),
+                                     m.shape[0]))
+    return out
+
+
+def trlwe_encrypt_u32(seed, index0, key, m, alpha):
+    m = np.ascontiguousarray(m, np.uint32).reshape(-1, LVL1_N)
+    out = np.zeros((m.shape[0], 2, LVL1_N), np.uint32)
+    check(lib().fhe_trlwe_encrypt_u32(seed, index0, _ptr(key), _ptr(m), alpha, _ptr(out),
+                                      m.shape[0]))
+    return out
+
+
+def trgsw_encrypt_u32(seed, index, key, m, alpha):
+    out = np.zeros((6, 2, LVL1_N), np.uint32)
+    check(lib().fhe_trgsw_encrypt_u32(seed, index, _ptr(key), int(m), alpha, _ptr(out)))
+    return out
+
+
+def tlwe_phase_u32(c, key):
+    """b - <a, s> mod 2^32 for a batch of TLWEs (numpy, exact)."""
+    c = np.atleast_2d(c).astype(np.uint64)
+    n = key.shape[0]
+    dot = (c[:, :n] * key.astype(np.uint64)[None, :]).sum(axis=1, dtype=np.uint64)
+    return ((c[:, n] - dot) & 0xFFFFFFFF).astype(np.uint32)
+
+
+def torus32_signed(x):
+    return np.asarray(x, np.uint32).view(np.int32).astype(np.int64)
+
+
+# ---------------------------------------------------------------- device side
+class DeviceBuffer:
+    """Device allocation owned by Python (fhe_malloc / fhe_free)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().fhe_malloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        buf = cls(a.nbytes)
+        check(lib().fhe_memcpy_h2d(buf.ptr, _ptr(a), a.nbytes))
+        return buf
+
+    def to_array(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib().fhe_memcpy_d2h(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().fhe_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One GPU, resident keys (fhe_ctx)."""
+
+    def __init__(self, params, device=0):
+        self.params = params
+        h = ctypes.c_void_p()
+        check(lib().fhe_ctx_create(ctypes.byref(params), device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().fhe_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self):
+        return lib().fhe_ctx_stream(self.h)
+
+    def sync(self):
+        check(lib().fhe_ctx_sync(self.h))
+
+    def load_bk(self, level, bk, fmt=BK_INT):
+        bk = np.ascontiguousarray(bk)
+        check(lib().fhe_load_bk(self.h, level, _ptr(bk), fmt))
+
+    def load_ksk(self, kind, ksk):
+        ksk = np.ascontiguousarray(ksk, np.uint32)
+        check(lib().fhe_load_ksk(self.h, kind, _ptr(ksk)))
+
+    def set_pbs_variant(self, variant):
+        check(lib().fhe_set_pbs_variant(self.h, variant))
+
+    # ---- host-array convenience forms (synchronous)
+    def pbs_lvl01(self, tlwe0, tv):
+        tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)
+        tv = np.ascontiguousarray(tv, np.uint32)
+        B = tlwe0.shape[0]
+        stride = 0 if tv.ndim == 1 else LVL1_N
+        out = np.zeros((B, LVL1_N + 1), np.uint32)
+        check(lib().fhe_pbs_lvl01(self.h, _ptr(tlwe0), _ptr(tv), stride, _ptr(out), B))
+        return out
+
+    def iks(self, kind, tlwe):
+        tlwe = np.ascontiguousarray(tlwe)
+        B = tlwe.shape[0]
+        tn = LVL1_N if kind == 21 else self.params.n0
+        out = np.zeros((B, tn + 1), np.uint32)
+        check(lib().fhe_iks(self.h, kind, _ptr(tlwe), _ptr(out), B))
+        return out
+
+    def _run_dev(self, fn, inputs, out_shape, out_dtype, *extra):
+        """Copy inputs to the device, run fn(ctx, *dev_in, dev_out, *extra, stream=None),
+        copy the output back."""
+        dins = [DeviceBuffer.from_array(a) for a in inputs]
+        dout = DeviceBuffer(int(np.prod(out_shape)) * np.dtype(out_dtype).itemsize)
+        check(fn(self.h, *[d.ptr for d in dins], dout.ptr, *extra, None))
+        self.sync()
+        out = dout.to_array(out_dtype, out_shape)
+        for d in dins:
+            d.free()
+        dout.free()
+        return out
+
+    def blind_rotate_lvl01(self, tlwe0, tv):
+        tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)
+        tv = np.ascontiguousarray(tv, np.uint32)
+        B = tlwe0.shape[0]
+        stride = 0 if tv.ndim == 1 else LVL1_N
+        dins = [DeviceBuffer.from_array(tlwe0), DeviceBuffer.from_array(tv)]
+        dout = DeviceBuffer(B * 2 * LVL1_N * 4)
+        check(lib().fhe_blind_rotate_lvl01_dev(self.h, dins[0].ptr, dins[1].ptr, stride, dout.ptr,
+                                               B, None))
+        self.sync()
+        return dout.to_array(np.uint32, (B, 2, LVL1_N))
+
+    def sample_extract_lvl1(self, trlwe, index=0):
+        trlwe = np.ascontiguousarray(trlwe, np.uint32)
+        B = trlwe.shape[0]
+        return self._run_dev(lib().fhe_sample_extract_lvl1_dev, [trlwe], (B, LVL1_N + 1),
+                             np.uint32, index, B)
+
+    def external_product_lvl1(self, trlwe, trgswfft):
+        trlwe = np.ascontiguousarray(trlwe, np.uint32)
+        B = trlwe.shape[0]
+        return self._run_dev(lib().fhe_external_product_lvl1_dev,
+                             [trlwe, np.ascontiguousarray(trgswfft, np.float64)],
+                             (B, 2, LVL1_N), np.uint32, B)
+
+    def ifft_lvl1(self, poly):
+        poly = np.ascontiguousarray(poly, np.uint32).reshape(-1, LVL1_N)
+        B = poly.shape[0]
+        return self._run_dev(lib().fhe_ifft_lvl1_dev, [poly], (B, LVL1_N), np.float64, B)
+
+    def fft_lvl1(self, fd):
+        fd = np.ascontiguousarray(fd, np.float64).reshape(-1, LVL1_N)
+        B = fd.shape[0]
+        return self._run_dev(lib().fhe_fft_lvl1_dev, [fd], (B, LVL1_N), np.uint32, B)
+
+    def gate(self, gate, a, b, result_type=LOGIC):
+        a = np.ascontiguousarray(a, np.uint32)
+        b = np.ascontiguousarray(b, np.uint32)
+        B = a.shape[0]
+        da, db = DeviceBuffer.from_array(a), DeviceBuffer.from_array(b)
+        dout = DeviceBuffer(a.nbytes)
+        check(lib().fhe_gate_lvl1_dev(self.h, gate, da.ptr, db.ptr, dout.ptr, result_type, B,
+                                      None))
+        self.sync()
+        return dout.to_array(np.uint32, a.shape)
+
+    def msb_gate(self, a, result_type=LOGIC):
+        a = np.ascontiguousarray(a, np.uint32)
+        return self._run_dev(lib().fhe_msb_gate_lvl1_dev, [a], a.shape, np.uint32, result_type,
+                             a.shape[0])
+
+    def ide_gate(self, a, scale_bits):
+        a = np.ascontiguousarray(a, np.uint32)
+        return self._run_dev(lib().fhe_ide_gate_lvl1_dev, [a], a.shape, np.uint32, scale_bits,
+                             a.shape[0])
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    check(lib().fhe_device_name(buf, 256))
+    return buf.value.decode()

@@ -1,0 +1,511 @@
+// capi.hip -- the extern "C" boundary: device context, key upload, batched
+// entry points, bootstrap operators and memory/timing helpers.
+//
+// A context corresponds to one cufft::CuFFT_Torus<P> pair of the reference
+// (src/cuHEDB/fft_gpu/cufft_gpu.cuh:10-451) but holds both levels, keeps
+// every key resident on the device, and takes whole batches per call instead
+// of one ciphertext per stream_id.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "fft64.hpp"
+#include "launch.hpp"
+#include "params.hpp"
+
+using namespace fhe;
+
+struct fhe_ctx {
+    fhe_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    cd *bk1 = nullptr;  // lvl01 key, device FD layout
+    cd *bk2 = nullptr;  // lvl02 key, device FD layout
+    uint32_t *ksk[3] = {nullptr, nullptr, nullptr};  // kinds 10, 20, 21
+    cd *trgsw1 = nullptr;  // external-product TRGSW in device layout
+    cd *trgsw2 = nullptr;
+    int variant = FHE_PBS_FP64;
+    // grow-on-demand scratch for the bootstrap operators
+    uint32_t *tv1 = nullptr;     // one lvl1 test vector
+    uint32_t *tmp1 = nullptr;    // B x 1025
+    size_t tmp1_cap = 0;
+    uint32_t *tmp0 = nullptr;    // B x (n0 + 1)
+    size_t tmp0_cap = 0;
+};
+
+namespace {
+
+int ksk_slot(int kind) { return kind == 10 ? 0 : kind == 20 ? 1 : kind == 21 ? 2 : -1; }
+
+int hip_fail(hipError_t e, const char *what)
+{
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? FHE_ERR_ALLOC : FHE_ERR_DEVICE;
+}
+
+#define HIP_TRY(expr, what)                                 \
+    do {                                                    \
+        hipError_t _e = (expr);                             \
+        if (_e != hipSuccess) return hip_fail(_e, what);    \
+    } while (0)
+
+hipStream_t pick(fhe_ctx *ctx, void *stream)
+{
+    return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+}
+
+int grow(uint32_t *&buf, size_t &cap, size_t words)
+{
+    if (cap >= words) return FHE_OK;
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    cap = 0;
+    HIP_TRY(hipMalloc(&buf, words * sizeof(uint32_t)), "scratch allocation");
+    cap = words;
+    return FHE_OK;
+}
+
+int ks_target_n(const fhe_ctx *ctx, int kind) { return kind == 21 ? Lvl1::N : int(ctx->p.n0); }
+int ks_domain_words(int kind) { return kind == 10 ? Lvl1::N + 1 : Lvl2::N + 1; }
+
+}  // namespace
+
+extern "C" {
+
+int fhe_ctx_create(const fhe_params *p, int device, fhe_ctx **out)
+{
+    FHE_CHECK_ARG(p && out, "fhe_ctx_create: null argument");
+    FHE_CHECK_ARG(p->n0 >= 1 && p->n0 <= uint32_t(kMaxN0), "fhe_ctx_create: n0 must be in [1, %d]",
+                  kMaxN0);
+    *out = nullptr;
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    FHE_CHECK_ARG(device >= 0 && device < count, "fhe_ctx_create: device %d of %d", device, count);
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    auto *ctx = new fhe_ctx;
+    ctx->p = *p;
+    ctx->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+    if (e == hipSuccess) e = hipMalloc(&ctx->tv1, Lvl1::N * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        fhe_ctx_destroy(ctx);
+        return hip_fail(e, "fhe_ctx_create");
+    }
+    *out = ctx;
+    return FHE_OK;
+}
+
+int fhe_ctx_destroy(fhe_ctx *ctx)
+{
+    if (!ctx) return FHE_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    void *bufs[] = {ctx->bk1, ctx->bk2, ctx->ksk[0], ctx->ksk[1], ctx->ksk[2], ctx->trgsw1,
+                    ctx->trgsw2, ctx->tv1, ctx->tmp1, ctx->tmp0};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FHE_OK;
+}
+
+int fhe_ctx_sync(fhe_ctx *ctx)
+{
+    FHE_CHECK_ARG(ctx, "fhe_ctx_sync: null context");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return FHE_OK;
+}
+
+void *fhe_ctx_stream(fhe_ctx *ctx) { return ctx ? static_cast<void *>(ctx->stream) : nullptr; }
+
+int fhe_set_pbs_variant(fhe_ctx *ctx, int variant)
+{
+    FHE_CHECK_ARG(ctx, "fhe_set_pbs_variant: null context");
+    FHE_CHECK_ARG(variant == FHE_PBS_FP64, "fhe_set_pbs_variant: unknown variant %d", variant);
+    ctx->variant = variant;
+    return FHE_OK;
+}
+
+int fhe_load_bk(fhe_ctx *ctx, int level, const void *bk_host, int format)
+{
+    FHE_CHECK_ARG(ctx && bk_host, "fhe_load_bk: null argument");
+    FHE_CHECK_ARG(level == 1 || level == 2, "fhe_load_bk: level must be 1 or 2");
+    FHE_CHECK_ARG(format == FHE_BK_INT || format == FHE_BK_FFT, "fhe_load_bk: bad format %d", format);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    const int n0 = int(ctx->p.n0);
+    if (level == 2) {
+        set_error("fhe_load_bk: level 2 is not available in this build");
+        return FHE_ERR_UNSUPPORTED;
+    }
+    const size_t elems = bk1_device_elems(n0);
+    if (!ctx->bk1) HIP_TRY(hipMalloc(&ctx->bk1, elems * sizeof(cd)), "BK allocation");
+    const size_t npoly = size_t(n0) * Lvl1::rows * 2;
+    const size_t src_bytes =
+        npoly * Lvl1::N * (format == FHE_BK_INT ? sizeof(uint32_t) : sizeof(double));
+    void *tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, src_bytes), "BK staging allocation");
+    hipError_t e = hipMemcpyAsync(tmp, bk_host, src_bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = format == FHE_BK_INT
+                ? launch_bk1_from_int(static_cast<uint32_t *>(tmp), ctx->bk1, n0, ctx->stream)
+                : launch_bk1_from_fft(static_cast<double *>(tmp), ctx->bk1, n0, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tmp);
+    HIP_TRY(e, "fhe_load_bk");
+    return FHE_OK;
+}
+
+int fhe_load_ksk(fhe_ctx *ctx, int kind, const uint32_t *ksk_host)
+{
+    FHE_CHECK_ARG(ctx && ksk_host, "fhe_load_ksk: null argument");
+    const int slot = ksk_slot(kind);
+    FHE_CHECK_ARG(slot >= 0, "fhe_load_ksk: kind must be 10, 20 or 21 (got %d)", kind);
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    const size_t bytes = fhe_ksk_words(&ctx->p, kind) * sizeof(uint32_t);
+    if (!ctx->ksk[slot]) HIP_TRY(hipMalloc(&ctx->ksk[slot], bytes), "KSK allocation");
+    HIP_TRY(hipMemcpy(ctx->ksk[slot], ksk_host, bytes, hipMemcpyHostToDevice), "KSK upload");
+    return FHE_OK;
+}
+
+int fhe_blind_rotate_lvl01_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint32_t *tv_dev,
+                               int tv_stride, uint32_t *acc_dev, int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && in_dev && tv_dev && acc_dev, "fhe_blind_rotate_lvl01_dev: null argument");
+    FHE_CHECK_ARG(B >= 0, "fhe_blind_rotate_lvl01_dev: negative batch");
+    FHE_CHECK_ARG(tv_stride == 0 || tv_stride == Lvl1::N, "tv_stride must be 0 or %d", Lvl1::N);
+    if (!ctx->bk1) {
+        set_error("fhe_blind_rotate_lvl01_dev: lvl01 bootstrapping key not loaded");
+        return FHE_ERR_NOT_LOADED;
+    }
+    HIP_TRY(launch_blind_rotate1(in_dev, int(ctx->p.n0), ctx->bk1, tv_dev, tv_stride, acc_dev,
+                                 false, B, pick(ctx, stream)),
+            "blind rotate lvl01");
+    return FHE_OK;
+}
+
+int fhe_pbs_lvl01_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint32_t *tv_dev, int tv_stride,
+                      uint32_t *out_dev, int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && in_dev && tv_dev && out_dev, "fhe_pbs_lvl01_dev: null argument");
+    FHE_CHECK_ARG(B >= 0, "fhe_pbs_lvl01_dev: negative batch");
+    FHE_CHECK_ARG(tv_stride == 0 || tv_stride == Lvl1::N, "tv_stride must be 0 or %d", Lvl1::N);
+    if (!ctx->bk1) {
+        set_error("fhe_pbs_lvl01_dev: lvl01 bootstrapping key not loaded");
+        return FHE_ERR_NOT_LOADED;
+    }
+    HIP_TRY(launch_blind_rotate1(in_dev, int(ctx->p.n0), ctx->bk1, tv_dev, tv_stride, out_dev,
+                                 true, B, pick(ctx, stream)),
+            "pbs lvl01");
+    return FHE_OK;
+}
+
+int fhe_pbs_lvl02_dev(fhe_ctx *ctx, const uint32_t *, const uint64_t *, int, uint64_t *, int,
+                      void *)
+{
+    FHE_CHECK_ARG(ctx, "fhe_pbs_lvl02_dev: null context");
+    set_error("fhe_pbs_lvl02_dev: level 2 is not available in this build");
+    return FHE_ERR_UNSUPPORTED;
+}
+
+int fhe_sample_extract_lvl1_dev(fhe_ctx *ctx, const uint32_t *trlwe_dev, uint32_t *tlwe_dev,
+                                int index, int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && trlwe_dev && tlwe_dev, "fhe_sample_extract_lvl1_dev: null argument");
+    FHE_CHECK_ARG(B >= 0 && index >= 0 && index < Lvl1::N, "fhe_sample_extract_lvl1_dev: bad size");
+    HIP_TRY(launch_sample_extract1(trlwe_dev, tlwe_dev, index, B, pick(ctx, stream)),
+            "sample extract lvl1");
+    return FHE_OK;
+}
+
+int fhe_sample_extract_lvl2_dev(fhe_ctx *ctx, const uint64_t *, uint64_t *, int, int, void *)
+{
+    FHE_CHECK_ARG(ctx, "fhe_sample_extract_lvl2_dev: null context");
+    set_error("fhe_sample_extract_lvl2_dev: level 2 is not available in this build");
+    return FHE_ERR_UNSUPPORTED;
+}
+
+int fhe_iks_dev(fhe_ctx *ctx, int kind, const void *in_dev, uint32_t *out_dev, int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && in_dev && out_dev, "fhe_iks_dev: null argument");
+    FHE_CHECK_ARG(B >= 0, "fhe_iks_dev: negative batch");
+    const int slot = ksk_slot(kind);
+    FHE_CHECK_ARG(slot >= 0, "fhe_iks_dev: kind must be 10, 20 or 21 (got %d)", kind);
+    if (!ctx->ksk[slot]) {
+        set_error("fhe_iks_dev: key-switching key %d not loaded", kind);
+        return FHE_ERR_NOT_LOADED;
+    }
+    HIP_TRY(launch_iks(kind, in_dev, ctx->ksk[slot], out_dev, ks_target_n(ctx, kind), B,
+                       pick(ctx, stream)),
+            "identity key switch");
+    return FHE_OK;
+}
+
+int fhe_external_product_lvl1_dev(fhe_ctx *ctx, const uint32_t *trlwe_dev,
+                                  const double *trgswfft_dev, uint32_t *out_dev, int B,
+                                  void *stream)
+{
+    FHE_CHECK_ARG(ctx && trlwe_dev && trgswfft_dev && out_dev,
+                  "fhe_external_product_lvl1_dev: null argument");
+    FHE_CHECK_ARG(B >= 0, "fhe_external_product_lvl1_dev: negative batch");
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    if (!ctx->trgsw1)
+        HIP_TRY(hipMalloc(&ctx->trgsw1, size_t(Lvl1::rows) * 2 * 512 * sizeof(cd)),
+                "TRGSW scratch allocation");
+    hipStream_t s = pick(ctx, stream);
+    HIP_TRY(launch_trgsw1_from_fft(trgswfft_dev, ctx->trgsw1, s), "TRGSW layout");
+    HIP_TRY(launch_extprod1(trlwe_dev, ctx->trgsw1, out_dev, B, s), "external product lvl1");
+    return FHE_OK;
+}
+
+int fhe_external_product_lvl2_dev(fhe_ctx *ctx, const uint64_t *, const double *, uint64_t *, int,
+                                  void *)
+{
+    FHE_CHECK_ARG(ctx, "fhe_external_product_lvl2_dev: null context");
+    set_error("fhe_external_product_lvl2_dev: level 2 is not available in this build");
+    return FHE_ERR_UNSUPPORTED;
+}
+
+int fhe_ifft_lvl1_dev(fhe_ctx *ctx, const uint32_t *poly_dev, double *fd_dev, int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && poly_dev && fd_dev && B >= 0, "fhe_ifft_lvl1_dev: bad argument");
+    HIP_TRY(launch_ifft1(poly_dev, fd_dev, B, pick(ctx, stream)), "ifft lvl1");
+    return FHE_OK;
+}
+
+int fhe_fft_lvl1_dev(fhe_ctx *ctx, const double *fd_dev, uint32_t *poly_dev, int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && poly_dev && fd_dev && B >= 0, "fhe_fft_lvl1_dev: bad argument");
+    HIP_TRY(launch_fft1(fd_dev, poly_dev, B, pick(ctx, stream)), "fft lvl1");
+    return FHE_OK;
+}
+
+int fhe_ifft_lvl2_dev(fhe_ctx *ctx, const uint64_t *, double *, int, void *)
+{
+    FHE_CHECK_ARG(ctx, "fhe_ifft_lvl2_dev: null context");
+    set_error("fhe_ifft_lvl2_dev: level 2 is not available in this build");
+    return FHE_ERR_UNSUPPORTED;
+}
+
+int fhe_fft_lvl2_dev(fhe_ctx *ctx, const double *, uint64_t *, int, void *)
+{
+    FHE_CHECK_ARG(ctx, "fhe_fft_lvl2_dev: null context");
+    set_error("fhe_fft_lvl2_dev: level 2 is not available in this build");
+    return FHE_ERR_UNSUPPORTED;
+}
+
+// ---- bootstrap operators --------------------------------------------------
+
+namespace {
+// c = sa*a + sb*b + off -> IKS lvl10 -> PBS(tv = const u) [-> body += post]
+int gate_pipeline(fhe_ctx *ctx, const uint32_t *a, const uint32_t *b, int sa, int sb, uint32_t off,
+                  uint32_t u, bool ide, uint32_t scale_bits, uint32_t post, uint32_t *out, int B,
+                  hipStream_t s)
+{
+    if (!ctx->bk1 || !ctx->ksk[0]) {
+        set_error("bootstrap operator: lvl01 BK and KSK 10 must be loaded");
+        return FHE_ERR_NOT_LOADED;
+    }
+    if (B == 0) return FHE_OK;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    int rc = grow(ctx->tmp1, ctx->tmp1_cap, size_t(B) * (Lvl1::N + 1));
+    if (rc) return rc;
+    rc = grow(ctx->tmp0, ctx->tmp0_cap, size_t(B) * (ctx->p.n0 + 1));
+    if (rc) return rc;
+    HIP_TRY(launch_tlwe1_lincomb(a, b ? b : a, sa, b ? sb : 0, off, ctx->tmp1, B, s), "lincomb");
+    HIP_TRY(launch_iks(10, ctx->tmp1, ctx->ksk[0], ctx->tmp0, int(ctx->p.n0), B, s), "IKS lvl10");
+    HIP_TRY(ide ? launch_tv1_ide(ctx->tv1, scale_bits, s) : launch_tv1_const(ctx->tv1, u, s),
+            "test vector");
+    HIP_TRY(launch_blind_rotate1(ctx->tmp0, int(ctx->p.n0), ctx->bk1, ctx->tv1, 0, out, true, B, s),
+            "pbs lvl01");
+    if (post) HIP_TRY(launch_tlwe1_add_body(out, post, B, s), "post offset");
+    return FHE_OK;
+}
+}  // namespace
+
+int fhe_gate_lvl1_dev(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint32_t *b_dev,
+                      uint32_t *out_dev, int result_type, int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && a_dev && b_dev && out_dev && B >= 0, "fhe_gate_lvl1_dev: bad argument");
+    const uint32_t mu = Lvl1::mu;
+    const bool logic = result_type != 0;
+    hipStream_t s = pick(ctx, stream);
+    switch (gate) {
+    case FHE_GATE_NAND:  // HomGate<-1,-1,+mu> + GateBootstrapping (TFHEpp gate.cpp:50-65)
+        return gate_pipeline(ctx, a_dev, b_dev, -1, -1, mu, mu, false, 0, 0, out_dev, B, s);
+    case FHE_GATE_NOR:
+        return gate_pipeline(ctx, a_dev, b_dev, -1, -1, 0u - mu, mu, false, 0, 0, out_dev, B, s);
+    case FHE_GATE_XNOR:
+        return gate_pipeline(ctx, a_dev, b_dev, -2, -2, 0u - 2 * mu, mu, false, 0, 0, out_dev, B, s);
+    case FHE_GATE_AND: {  // cuHEDB::HomAND: a + b - mu/2, tv = offset
+        const uint32_t u = logic ? mu : mu << 1;
+        return gate_pipeline(ctx, a_dev, b_dev, 1, 1, 0u - (mu >> 1), u, false, 0, logic ? 0 : u,
+                             out_dev, B, s);
+    }
+    case FHE_GATE_OR: {  // cuHEDB::HomOR: a + b + mu/2
+        const uint32_t u = logic ? mu : mu << 1;
+        return gate_pipeline(ctx, a_dev, b_dev, 1, 1, mu >> 1, u, false, 0, logic ? 0 : u, out_dev,
+                             B, s);
+    }
+    default:
+        set_error("fhe_gate_lvl1_dev: unknown gate %d", gate);
+        return FHE_ERR_ARG;
+    }
+}
+
+int fhe_msb_gate_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_dev, int result_type,
+                          int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && in_dev && out_dev && B >= 0, "fhe_msb_gate_lvl1_dev: bad argument");
+    // cuHEDB::MSBGateBootstrapping: +2^26 offset, tv = -u, ARITHMETIC adds u
+    const uint32_t u = result_type ? Lvl1::mu : Lvl1::mu << 1;
+    return gate_pipeline(ctx, in_dev, nullptr, 1, 0, 1u << 26, 0u - u, false, 0,
+                         result_type ? 0 : u, out_dev, B, pick(ctx, stream));
+}
+
+int fhe_ide_gate_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_dev,
+                          uint32_t scale_bits, int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && in_dev && out_dev && B >= 0, "fhe_ide_gate_lvl1_dev: bad argument");
+    FHE_CHECK_ARG(scale_bits < 32, "fhe_ide_gate_lvl1_dev: scale_bits must be < 32");
+    // cuHEDB::IdeGateBootstrapping: +2^26 offset, tv = gpolygen(4, scale_bits)
+    return gate_pipeline(ctx, in_dev, nullptr, 1, 0, 1u << 26, 0, true, scale_bits, 0, out_dev, B,
+                         pick(ctx, stream));
+}
+
+// ---- host-pointer forms ---------------------------------------------------
+
+int fhe_pbs_lvl01(fhe_ctx *ctx, const uint32_t *in, const uint32_t *tv, int tv_stride,
+                  uint32_t *out, int B)
+{
+    FHE_CHECK_ARG(ctx && in && tv && out && B >= 0, "fhe_pbs_lvl01: bad argument");
+    FHE_CHECK_ARG(tv_stride == 0 || tv_stride == Lvl1::N, "tv_stride must be 0 or %d", Lvl1::N);
+    if (B == 0) return FHE_OK;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    const size_t in_b = size_t(B) * (ctx->p.n0 + 1) * 4, out_b = size_t(B) * (Lvl1::N + 1) * 4;
+    const size_t tv_b = (tv_stride ? size_t(B) : 1) * Lvl1::N * 4;
+    void *din = nullptr, *dtv = nullptr, *dout = nullptr;
+    hipError_t e = hipMalloc(&din, in_b);
+    if (e == hipSuccess) e = hipMalloc(&dtv, tv_b);
+    if (e == hipSuccess) e = hipMalloc(&dout, out_b);
+    if (e == hipSuccess) e = hipMemcpyAsync(din, in, in_b, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dtv, tv, tv_b, hipMemcpyHostToDevice, ctx->stream);
+    int rc = FHE_OK;
+    if (e == hipSuccess)
+        rc = fhe_pbs_lvl01_dev(ctx, static_cast<uint32_t *>(din), static_cast<uint32_t *>(dtv),
+                               tv_stride, static_cast<uint32_t *>(dout), B, nullptr);
+    if (e == hipSuccess && rc == FHE_OK)
+        e = hipMemcpyAsync(out, dout, out_b, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (din) (void)hipFree(din);
+    if (dtv) (void)hipFree(dtv);
+    if (dout) (void)hipFree(dout);
+    if (rc != FHE_OK) return rc;
+    HIP_TRY(e, "fhe_pbs_lvl01");
+    return FHE_OK;
+}
+
+int fhe_iks(fhe_ctx *ctx, int kind, const void *in, uint32_t *out, int B)
+{
+    FHE_CHECK_ARG(ctx && in && out && B >= 0, "fhe_iks: bad argument");
+    const int slot = ksk_slot(kind);
+    FHE_CHECK_ARG(slot >= 0, "fhe_iks: kind must be 10, 20 or 21 (got %d)", kind);
+    if (B == 0) return FHE_OK;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    const size_t in_b = size_t(B) * ks_domain_words(kind) * (kind == 10 ? 4 : 8);
+    const size_t out_b = size_t(B) * (ks_target_n(ctx, kind) + 1) * 4;
+    void *din = nullptr, *dout = nullptr;
+    hipError_t e = hipMalloc(&din, in_b);
+    if (e == hipSuccess) e = hipMalloc(&dout, out_b);
+    if (e == hipSuccess) e = hipMemcpyAsync(din, in, in_b, hipMemcpyHostToDevice, ctx->stream);
+    int rc = FHE_OK;
+    if (e == hipSuccess) rc = fhe_iks_dev(ctx, kind, din, static_cast<uint32_t *>(dout), B, nullptr);
+    if (e == hipSuccess && rc == FHE_OK)
+        e = hipMemcpyAsync(out, dout, out_b, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (din) (void)hipFree(din);
+    if (dout) (void)hipFree(dout);
+    if (rc != FHE_OK) return rc;
+    HIP_TRY(e, "fhe_iks");
+    return FHE_OK;
+}
+
+// ---- memory / timing helpers ---------------------------------------------
+
+int fhe_malloc(void **ptr, size_t bytes)
+{
+    FHE_CHECK_ARG(ptr, "fhe_malloc: null pointer");
+    *ptr = nullptr;
+    HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1), "hipMalloc");
+    return FHE_OK;
+}
+
+int fhe_free(void *ptr)
+{
+    if (ptr) HIP_TRY(hipFree(ptr), "hipFree");
+    return FHE_OK;
+}
+
+int fhe_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    FHE_CHECK_ARG(dst && src, "fhe_memcpy_h2d: null pointer");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    return FHE_OK;
+}
+
+int fhe_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    FHE_CHECK_ARG(dst && src, "fhe_memcpy_d2h: null pointer");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    return FHE_OK;
+}
+
+int fhe_memset_dev(void *dst, int value, size_t bytes)
+{
+    FHE_CHECK_ARG(dst, "fhe_memset_dev: null pointer");
+    HIP_TRY(hipMemset(dst, value, bytes), "hipMemset");
+    return FHE_OK;
+}
+
+int fhe_device_sync(void)
+{
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    return FHE_OK;
+}
+
+int fhe_timer_start(fhe_ctx *ctx, void *stream)
+{
+    FHE_CHECK_ARG(ctx, "fhe_timer_start: null context");
+    HIP_TRY(hipEventRecord(ctx->ev0, pick(ctx, stream)), "hipEventRecord");
+    return FHE_OK;
+}
+
+int fhe_timer_stop(fhe_ctx *ctx, void *stream, float *ms)
+{
+    FHE_CHECK_ARG(ctx && ms, "fhe_timer_stop: null argument");
+    HIP_TRY(hipEventRecord(ctx->ev1, pick(ctx, stream)), "hipEventRecord");
+    HIP_TRY(hipEventSynchronize(ctx->ev1), "hipEventSynchronize");
+    HIP_TRY(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
+    return FHE_OK;
+}
+
+int fhe_device_name(char *buf, int len)
+{
+    FHE_CHECK_ARG(buf && len > 0, "fhe_device_name: bad buffer");
+    hipDeviceProp_t prop;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    HIP_TRY(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    snprintf(buf, size_t(len), "%s (%s, %d CUs)", prop.name, prop.gcnArchName,
+             prop.multiProcessorCount);
+    return FHE_OK;
+}
+
+}  // extern "C"

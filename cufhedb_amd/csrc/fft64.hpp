@@ -1,0 +1,257 @@
+// fft64.hpp -- FP64 twisted negacyclic FFT of N = 1024 (512 complex points)
+// for one wave64 on gfx950.
+//
+// Mathematical map (same as the reference's TwistIFFT / TwistFFT,
+// thirdparty/TFHEpp/include/mulfft.hpp:58-121 over spqlios, and cuFHEDB's
+// ifft1024 / fft1024, src/cuHEDB/fft_gpu/fft_gpu.cuh:471-799):
+//   forward:  z_n = a_n + i a_{n+512},  Z[m] = sum_n z_n w^{n(4m+1)},
+//             w = exp(i pi / 1024), m, n in [0, 512)
+//   inverse:  z_n = (1/512) w^{-n} sum_m Z[m] exp(-2 pi i n m / 512)
+//
+// Layout and schedule (this build's own design):
+//   time domain : lane L, register j holds z[L + 64 j]            (natural)
+//   freq domain : lane s, register q holds Z[(s>>3) + 8 (s&7) + 64 q]
+// 512 = 8 x 8 x 8.  Pass 1 is a radix-8 DFT over the 8 registers of a lane;
+// the twist w^n is folded into the constants c_j = w^{64 j} before it and
+// the per-lane twiddle w^{L(4k+1)} after it.  Two transposes through a
+// 8 x 65 double2 LDS tile (row pad of one element keeps every aligned group
+// of 8 lanes on 32 distinct banks for ds_read/write_b128) regroup the data
+// for pass 2 (radix 8, twiddle W64^{L0 p}) and pass 3 (radix 8).  The
+// inverse runs the same steps mirrored with conjugate twiddles, so it takes
+// the permuted frequency layout and returns natural time order.
+// Twiddle powers are generated in registers from three per-lane bases, so
+// the transform reads no twiddle tables from memory.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fhe {
+
+struct cd {
+    double x, y;
+};
+
+__device__ __forceinline__ cd cadd(cd a, cd b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cd csub(cd a, cd b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cd cmul(cd a, cd b)
+{
+    return {__fma_rn(a.x, b.x, -a.y * b.y), __fma_rn(a.x, b.y, a.y * b.x)};
+}
+// a * conj(b)
+__device__ __forceinline__ cd cmulc(cd a, cd b)
+{
+    return {__fma_rn(a.x, b.x, a.y * b.y), __fma_rn(a.y, b.x, -a.x * b.y)};
+}
+// acc += a * b
+__device__ __forceinline__ void cmac(cd &acc, cd a, cd b)
+{
+    acc.x = __fma_rn(a.x, b.x, acc.x);
+    acc.x = __fma_rn(-a.y, b.y, acc.x);
+    acc.y = __fma_rn(a.x, b.y, acc.y);
+    acc.y = __fma_rn(a.y, b.x, acc.y);
+}
+
+constexpr double kR2 = 0.70710678118654752440;  // sqrt(1/2)
+
+// x * (S i)
+template <int S>
+__device__ __forceinline__ cd mul_si(cd a)
+{
+    return S > 0 ? cd{-a.y, a.x} : cd{a.y, -a.x};
+}
+
+// in-register DFT of size 4, X[k] = sum_j a_j (S i)^{jk}
+template <int S>
+__device__ __forceinline__ void dft4(cd &a0, cd &a1, cd &a2, cd &a3)
+{
+    cd p0 = cadd(a0, a2), p1 = csub(a0, a2);
+    cd q0 = cadd(a1, a3), q1 = mul_si<S>(csub(a1, a3));
+    a0 = cadd(p0, q0);
+    a1 = cadd(p1, q1);
+    a2 = csub(p0, q0);
+    a3 = csub(p1, q1);
+}
+
+// in-register DFT of size 8, natural order in and out, X[k] = sum_j v_j W8^{S jk}
+template <int S>
+__device__ __forceinline__ void dft8(cd v[8])
+{
+    cd a0 = cadd(v[0], v[4]), a1 = cadd(v[1], v[5]), a2 = cadd(v[2], v[6]), a3 = cadd(v[3], v[7]);
+    cd b0 = csub(v[0], v[4]);
+    cd t1 = csub(v[1], v[5]);
+    cd t2 = csub(v[2], v[6]);
+    cd t3 = csub(v[3], v[7]);
+    // W8^S = (1 + S i)/sqrt2, W8^{2S} = S i, W8^{3S} = (-1 + S i)/sqrt2
+    cd b1 = S > 0 ? cd{kR2 * (t1.x - t1.y), kR2 * (t1.x + t1.y)}
+                  : cd{kR2 * (t1.x + t1.y), kR2 * (t1.y - t1.x)};
+    cd b2 = mul_si<S>(t2);
+    cd b3 = S > 0 ? cd{kR2 * (-t3.x - t3.y), kR2 * (t3.x - t3.y)}
+                  : cd{kR2 * (t3.y - t3.x), kR2 * (-t3.x - t3.y)};
+    dft4<S>(a0, a1, a2, a3);
+    dft4<S>(b0, b1, b2, b3);
+    v[0] = a0; v[2] = a1; v[4] = a2; v[6] = a3;
+    v[1] = b0; v[3] = b1; v[5] = b2; v[7] = b3;
+}
+
+// c_j = exp(2 pi i j / 32), j = 0..7 (the w^{64 j} part of the twist)
+__device__ __forceinline__ cd twist_const(int j)
+{
+    constexpr double c1 = 0.98078528040323044913, s1 = 0.19509032201612826785;
+    constexpr double c2 = 0.92387953251128675613, s2 = 0.38268343236508977173;
+    constexpr double c3 = 0.83146961230254523708, s3 = 0.55557023301960222474;
+    switch (j) {
+    case 0: return {1.0, 0.0};
+    case 1: return {c1, s1};
+    case 2: return {c2, s2};
+    case 3: return {c3, s3};
+    case 4: return {kR2, kR2};
+    case 5: return {s3, c3};
+    case 6: return {s2, c2};
+    default: return {s1, c1};
+    }
+}
+
+// Per-lane twiddle bases (computed once per kernel):
+//   wl  = w^L            (pass-1 twist, w = exp(i pi/1024))
+//   w4l = w^{4L}         (pass-1 step: W512^L)
+//   w64 = W64^{L & 7}    (pass-2 twiddle base, lane role L0 = lane & 7)
+struct Twiddles1024 {
+    cd wl, w4l, w64;
+};
+
+__device__ __forceinline__ cd unit_root(double num, double den)
+{
+    // exp(i pi num/den)
+    double s, c;
+    sincospi(num / den, &s, &c);
+    return {c, s};
+}
+
+__device__ __forceinline__ Twiddles1024 make_twiddles1024(int lane)
+{
+    Twiddles1024 t;
+    t.wl = unit_root(double(lane), 1024.0);
+    t.w4l = unit_root(double(lane), 256.0);
+    t.w64 = unit_root(double(lane & 7), 32.0);
+    return t;
+}
+
+constexpr int kFftPad = 65;                 // LDS row stride in double2
+constexpr int kFftLdsElems = 8 * kFftPad;   // 520 double2 = 8320 bytes
+
+// Forward transform: v[j] = z[L + 64 j] on entry, Z in permuted layout on exit.
+__device__ __forceinline__ void fft512_fwd(cd v[8], cd *lds, int lane, const Twiddles1024 &tw)
+{
+    // twist constants, pass 1
+#pragma unroll
+    for (int j = 1; j < 8; j++) v[j] = cmul(v[j], twist_const(j));
+    dft8<1>(v);
+    {
+        cd t = tw.wl;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            v[k] = cmul(v[k], t);
+            if (k < 7) t = cmul(t, tw.w4l);
+        }
+    }
+    // transpose 1: T[k1][L] -> lane (L0 + 8 k1) register L1
+#pragma unroll
+    for (int k = 0; k < 8; k++) lds[k * kFftPad + lane] = v[k];
+    __syncthreads();
+    {
+        const int L0 = lane & 7, k1 = lane >> 3;
+#pragma unroll
+        for (int l1 = 0; l1 < 8; l1++) v[l1] = lds[k1 * kFftPad + L0 + 8 * l1];
+    }
+    __syncthreads();
+    dft8<1>(v);
+    {
+        cd t = tw.w64;
+        cd p = t;
+#pragma unroll
+        for (int k = 1; k < 8; k++) {
+            v[k] = cmul(v[k], p);
+            if (k < 7) p = cmul(p, t);
+        }
+    }
+    // transpose 2: T[p][lane] -> lane (p + 8 k1) register L0
+#pragma unroll
+    for (int k = 0; k < 8; k++) lds[k * kFftPad + lane] = v[k];
+    __syncthreads();
+    {
+        const int p = lane & 7, k1 = lane >> 3;
+#pragma unroll
+        for (int l0 = 0; l0 < 8; l0++) v[l0] = lds[p * kFftPad + l0 + 8 * k1];
+    }
+    __syncthreads();
+    dft8<1>(v);
+}
+
+// Inverse transform (unnormalised: returns 512 * z): permuted Z in, natural z out.
+__device__ __forceinline__ void fft512_inv(cd v[8], cd *lds, int lane, const Twiddles1024 &tw)
+{
+    dft8<-1>(v);
+    {
+        const int p = lane & 7, k1 = lane >> 3;
+#pragma unroll
+        for (int l0 = 0; l0 < 8; l0++) lds[p * kFftPad + l0 + 8 * k1] = v[l0];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = lds[k * kFftPad + lane];
+    __syncthreads();
+    {
+        cd t = tw.w64;
+        cd p = t;
+#pragma unroll
+        for (int k = 1; k < 8; k++) {
+            v[k] = cmulc(v[k], p);
+            if (k < 7) p = cmul(p, t);
+        }
+    }
+    dft8<-1>(v);
+    {
+        const int L0 = lane & 7, k1 = lane >> 3;
+#pragma unroll
+        for (int l1 = 0; l1 < 8; l1++) lds[k1 * kFftPad + L0 + 8 * l1] = v[l1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = lds[k * kFftPad + lane];
+    __syncthreads();
+    {
+        cd t = tw.wl;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            v[k] = cmulc(v[k], t);
+            if (k < 7) t = cmul(t, tw.w4l);
+        }
+    }
+    dft8<-1>(v);
+#pragma unroll
+    for (int j = 1; j < 8; j++) v[j] = cmulc(v[j], twist_const(j));
+}
+
+// frequency index held by (lane s, register q) in the permuted layout
+__host__ __device__ constexpr int fd_index1024(int s, int q) { return (s >> 3) + 8 * (s & 7) + 64 * q; }
+
+// bit reversal over 9 bits: spqlios order of the 512 FD points
+__host__ __device__ constexpr int rev9(int x)
+{
+    int r = 0;
+    for (int i = 0; i < 9; i++) r |= ((x >> i) & 1) << (8 - i);
+    return r;
+}
+
+// uint32_t(int64_t(x)) for |x| < 2^53 (TwistFFT truncation,
+// fft_processor_spqlios.cpp:121), computed exactly in FP64.
+__device__ __forceinline__ uint32_t trunc_torus32(double x)
+{
+    double t = trunc(x);
+    double q = floor(t * 0x1p-32);
+    double r = __fma_rn(-q, 0x1p32, t);
+    return uint32_t(r);
+}
+
+}  // namespace fhe

@@ -1,0 +1,146 @@
+// iks.hip -- batched identity key switching and small element-wise kernels.
+//
+// IdentityKeySwitch (thirdparty/TFHEpp/src/keyswitch.cpp:12-48):
+//   b'   = b                         (same torus width)
+//        = (b + 2^(W-33)) >> (W-32)  (64-bit domain -> 32-bit target)
+//   for i < dn, j < t:  a_ij = ((a_i + 2^(W-1-t*basebit)) >> (W-(j+1)basebit)) & mask
+//                       if a_ij != 0: res -= ksk[i][j][a_ij - 1]
+// The reference runs this on the host CPU before every bootstrap
+// (src/cuHEDB/gatebootstrap_gpu.cuh:42,59,...).  Here one workgroup switches
+// one ciphertext: the dn*t digits are computed once into LDS, then each
+// thread owns output columns and walks the selected key rows, so every row
+// read is a coalesced run of (tn+1) words.  The key stays in HBM in the
+// reference's own layout [dn][t][2^basebit - 1][tn + 1].
+#include "launch.hpp"
+#include "params.hpp"
+
+namespace fhe {
+
+namespace {
+
+template <class TD, int DN, int T, int BB>
+__global__ __launch_bounds__(256) void k_iks(const TD *__restrict__ in,
+                                             const uint32_t *__restrict__ ksk,
+                                             uint32_t *__restrict__ out, int tn, int B)
+{
+    constexpr int W = int(sizeof(TD) * 8);
+    constexpr int NDIG = DN * T;
+    constexpr uint32_t mask = (1u << BB) - 1;
+    constexpr size_t nd = (size_t(1) << BB) - 1;
+    __shared__ uint16_t dig[NDIG];
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const TD *c = in + size_t(b) * (DN + 1);
+    const TD prec = TD(1) << (W - (1 + BB * T));
+    for (int e = threadIdx.x; e < NDIG; e += blockDim.x) {
+        const int i = e / T, j = e % T;
+        const TD abar = c[i] + prec;
+        dig[e] = uint16_t((abar >> (W - (j + 1) * BB)) & mask);
+    }
+    uint32_t body;
+    if constexpr (W == 32)
+        body = uint32_t(c[DN]);
+    else
+        body = uint32_t((c[DN] + (TD(1) << (W - 33))) >> (W - 32));
+    __syncthreads();
+    const size_t row = size_t(tn) + 1;
+    uint32_t *o = out + size_t(b) * row;
+    for (int k = threadIdx.x; k <= tn; k += blockDim.x) {
+        uint32_t acc = (k == tn) ? body : 0u;
+        const uint32_t *kc = ksk + k;
+#pragma unroll 8
+        for (int e = 0; e < NDIG; e++) {
+            const uint32_t d = dig[e];
+            if (d != 0) acc -= kc[(size_t(e) * nd + (d - 1)) * row];
+        }
+        o[k] = acc;
+    }
+}
+
+__global__ void k_tlwe1_lincomb(const uint32_t *__restrict__ a, const uint32_t *__restrict__ b,
+                                int sa, int sb, uint32_t offset, uint32_t *__restrict__ out,
+                                int B)
+{
+    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    constexpr int W = Lvl1::N + 1;
+    if (gid >= size_t(B) * W) return;
+    uint32_t v = uint32_t(sa) * a[gid] + uint32_t(sb) * b[gid];
+    if (gid % W == Lvl1::N) v += offset;
+    out[gid] = v;
+}
+
+__global__ void k_tlwe1_add_body(uint32_t *__restrict__ tlwe, uint32_t add, int B)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < B) tlwe[size_t(b) * (Lvl1::N + 1) + Lvl1::N] += add;
+}
+
+__global__ void k_tv1_const(uint32_t *tv, uint32_t u)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < Lvl1::N) tv[i] = u;
+}
+
+// gpolygen (src/HEDB/comparison/tfhepp_utils.h:113-120): 2^scale * (i >> 6)
+__global__ void k_tv1_ide(uint32_t *tv, uint32_t scale_bits)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < Lvl1::N) tv[i] = (1u << scale_bits) * uint32_t(i >> 6);
+}
+
+}  // namespace
+
+hipError_t launch_iks(int kind, const void *in, const uint32_t *ksk, uint32_t *out, int tn, int B,
+                      hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    switch (kind) {
+    case 10:
+        hipLaunchKernelGGL((k_iks<uint32_t, Lvl1::N, KS10::t, KS10::basebit>), dim3(B), dim3(256),
+                           0, s, static_cast<const uint32_t *>(in), ksk, out, tn, B);
+        break;
+    case 20:
+        hipLaunchKernelGGL((k_iks<uint64_t, Lvl2::N, KS20::t, KS20::basebit>), dim3(B), dim3(256),
+                           0, s, static_cast<const uint64_t *>(in), ksk, out, tn, B);
+        break;
+    case 21:
+        hipLaunchKernelGGL((k_iks<uint64_t, Lvl2::N, KS21::t, KS21::basebit>), dim3(B), dim3(256),
+                           0, s, static_cast<const uint64_t *>(in), ksk, out, tn, B);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_tlwe1_lincomb(const uint32_t *a, const uint32_t *b, int sa, int sb,
+                                uint32_t offset, uint32_t *out, int B, hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    const size_t n = size_t(B) * (Lvl1::N + 1);
+    hipLaunchKernelGGL(k_tlwe1_lincomb, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, a, b, sa,
+                       sb, offset, out, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_tlwe1_add_body(uint32_t *tlwe, uint32_t add, int B, hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tlwe1_add_body, dim3(unsigned((B + 255) / 256)), dim3(256), 0, s, tlwe,
+                       add, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_tv1_const(uint32_t *tv, uint32_t u, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tv1_const, dim3(Lvl1::N / 256), dim3(256), 0, s, tv, u);
+    return hipGetLastError();
+}
+
+hipError_t launch_tv1_ide(uint32_t *tv, uint32_t scale_bits, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tv1_ide, dim3(Lvl1::N / 256), dim3(256), 0, s, tv, scale_bits);
+    return hipGetLastError();
+}
+
+}  // namespace fhe

@@ -1,0 +1,57 @@
+// launch.hpp -- host launch wrappers of the device kernels (one .hip file per
+// kernel family); all take device pointers and a stream and only enqueue.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fhe {
+
+struct cd;
+
+// ---- lvl1 (N = 1024, 32-bit torus), pbs_lvl1.hip ------------------------
+// device bootstrapping-key layout: [n0][6 rows][2 cols][8 q][64 lanes] complex
+size_t bk1_device_elems(int n0);
+hipError_t launch_bk1_from_int(const uint32_t *bk_int, cd *bk_dev, int n0, hipStream_t s);
+hipError_t launch_bk1_from_fft(const double *bk_fft, cd *bk_dev, int n0, hipStream_t s);
+// out: SE ? B x 1025 TLWE : B x 2048 TRLWE (accumulator)
+hipError_t launch_blind_rotate1(const uint32_t *in, int n0, const cd *bk, const uint32_t *tv,
+                                int tv_stride, uint32_t *out, bool sample_extract, int B,
+                                hipStream_t s);
+hipError_t launch_trgsw1_from_fft(const double *trgswfft, cd *trgsw_dev, hipStream_t s);
+hipError_t launch_extprod1(const uint32_t *trlwe, const cd *trgsw_dev, uint32_t *out, int B,
+                           hipStream_t s);
+hipError_t launch_ifft1(const uint32_t *poly, double *fd, int B, hipStream_t s);
+hipError_t launch_fft1(const double *fd, uint32_t *poly, int B, hipStream_t s);
+hipError_t launch_sample_extract1(const uint32_t *trlwe, uint32_t *tlwe, int index, int B,
+                                  hipStream_t s);
+
+// ---- lvl2 (N = 2048, 64-bit torus), pbs_lvl2.hip ------------------------
+size_t bk2_device_elems(int n0);
+hipError_t launch_bk2_from_int(const uint64_t *bk_int, cd *bk_dev, int n0, hipStream_t s);
+hipError_t launch_bk2_from_fft(const double *bk_fft, cd *bk_dev, int n0, hipStream_t s);
+hipError_t launch_blind_rotate2(const uint32_t *in, int n0, const cd *bk, const uint64_t *tv,
+                                int tv_stride, uint64_t *out, bool sample_extract, int B,
+                                hipStream_t s);
+hipError_t launch_trgsw2_from_fft(const double *trgswfft, cd *trgsw_dev, hipStream_t s);
+hipError_t launch_extprod2(const uint64_t *trlwe, const cd *trgsw_dev, uint64_t *out, int B,
+                           hipStream_t s);
+hipError_t launch_ifft2(const uint64_t *poly, double *fd, int B, hipStream_t s);
+hipError_t launch_fft2(const double *fd, uint64_t *poly, int B, hipStream_t s);
+hipError_t launch_sample_extract2(const uint64_t *trlwe, uint64_t *tlwe, int index, int B,
+                                  hipStream_t s);
+
+// ---- identity key switching and small element-wise ops, iks.hip ---------
+// kind 10: uint32 domain (dn = 1024), kind 20/21: uint64 domain (dn = 2048)
+hipError_t launch_iks(int kind, const void *in, const uint32_t *ksk, uint32_t *out, int tn, int B,
+                      hipStream_t s);
+// out = sa*a + sb*b (+ offset on the body word), lvl1 TLWEs of 1025 words
+hipError_t launch_tlwe1_lincomb(const uint32_t *a, const uint32_t *b, int sa, int sb,
+                                uint32_t offset, uint32_t *out, int B, hipStream_t s);
+// tlwe[1024] += add for B lvl1 TLWEs
+hipError_t launch_tlwe1_add_body(uint32_t *tlwe, uint32_t add, int B, hipStream_t s);
+// test vectors: constant u, or gpolygen (scale 2^scale_bits, padding 6)
+hipError_t launch_tv1_const(uint32_t *tv, uint32_t u, hipStream_t s);
+hipError_t launch_tv1_ide(uint32_t *tv, uint32_t scale_bits, hipStream_t s);
+
+}  // namespace fhe

@@ -1,0 +1,367 @@
+// pbs_lvl1.hip -- lvl1 (N = 1024, 32-bit torus) kernels for gfx950, FP64 path.
+//
+// One wave64 owns one ciphertext for the whole blind rotation: its
+// accumulator (2 x 1024 words) lives in LDS, the FFT exchanges go through a
+// private 8 KB LDS tile, and the frequency-domain accumulators of the
+// external product stay in registers.  A launch bootstraps a whole batch
+// (one workgroup per ciphertext); every wave streams BK[i] (96 KB per CMUX)
+// from L2 / Infinity Cache, where the 62-66 MB key stays resident across
+// the batch.
+//
+// Reference semantics:
+//   BlindRotate            thirdparty/TFHEpp/include/gatebootstrapping.hpp:15-43
+//   CMUX with X^a - 1      thirdparty/TFHEpp/src/detwfa.cpp:21-30
+//   external product       thirdparty/TFHEpp/src/trgsw.cpp:78-103
+//   decomposition          thirdparty/TFHEpp/src/trgsw.cpp:9-39
+//   SampleExtractIndex     thirdparty/TFHEpp/src/trlwe.cpp:109-118
+//   GPU counterpart        src/cuHEDB/fft_gpu/BlindRotate_gpu.cuh:401-531
+// The a-bar == 0 skip of the reference is kept (it is wave-uniform here).
+#include "fft64.hpp"
+#include "launch.hpp"
+#include "params.hpp"
+
+namespace fhe {
+
+namespace {
+
+constexpr int N1 = Lvl1::N;
+constexpr uint32_t kDecOff1 = decomp_total_offset<uint32_t, 32, Lvl1::l, Lvl1::Bgbit>();
+constexpr int kPolyCd = 512;  // complex points per polynomial
+
+// (X^a - 1) * poly at coefficient n, a in [0, 2N)
+__device__ __forceinline__ uint32_t rot_minus_one(const uint32_t *poly, uint32_t a, int n)
+{
+    uint32_t idx = (uint32_t(n) - a) & (2u * N1 - 1);
+    uint32_t v = poly[idx & (N1 - 1)];
+    if (idx & N1) v = 0u - v;
+    return v - poly[n];
+}
+
+// signed gadget digit d of x (DecompositionPolynomial), as a double
+__device__ __forceinline__ double digit1(uint32_t x, int d)
+{
+    const int sh = 32 - Lvl1::Bgbit * (d + 1);
+    return double(int32_t((x + kDecOff1) >> sh & ((1u << Lvl1::Bgbit) - 1)) -
+                  (1 << (Lvl1::Bgbit - 1)));
+}
+
+// load one TRGSW row (both output columns) of the device key layout
+__device__ __forceinline__ void load_row(const cd *row, int lane, cd b0[8], cd b1[8])
+{
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        b0[q] = row[q * 64 + lane];
+        b1[q] = row[8 * 64 + q * 64 + lane];
+    }
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_blind_rotate1(const uint32_t *__restrict__ in, int n0,
+                                                      const cd *__restrict__ bk,
+                                                      const uint32_t *__restrict__ tv,
+                                                      int tv_stride, uint32_t *__restrict__ out,
+                                                      int sample_extract, int B)
+{
+    __shared__ uint32_t acc[2 * N1];
+    __shared__ cd tile[kFftLdsElems];
+    const int lane = threadIdx.x;
+    const int ct = blockIdx.x;
+    if (ct >= B) return;
+    const uint32_t *c0 = in + size_t(ct) * (n0 + 1);
+    const uint32_t *tvp = tv + size_t(ct) * tv_stride;
+    const Twiddles1024 tw = make_twiddles1024(lane);
+
+    // acc = (0, X^bbar * tv), bbar = 2N - (b >> 21)   (no rounding)
+    const uint32_t bbar = 2u * N1 - (c0[n0] >> (32 - 1 - Lvl1::nbit));
+#pragma unroll
+    for (int k = 0; k < N1 / 64; k++) {
+        const int n = lane + 64 * k;
+        acc[n] = 0;
+        uint32_t idx = (uint32_t(n) - bbar) & (2u * N1 - 1);
+        uint32_t v = tvp[idx & (N1 - 1)];
+        acc[N1 + n] = (idx & N1) ? 0u - v : v;
+    }
+    __syncthreads();
+
+    constexpr uint32_t kRound = 1u << (32 - 2 - Lvl1::nbit);
+    for (int i = 0; i < n0; i++) {
+        const uint32_t abar = (c0[i] + kRound) >> (32 - 1 - Lvl1::nbit);
+        if (abar == 0) continue;
+        const cd *bki = bk + size_t(i) * (Lvl1::rows * 2 * kPolyCd);
+        cd fd0[8], fd1[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) fd0[q] = fd1[q] = cd{0.0, 0.0};
+#pragma unroll 1
+        for (int c = 0; c < 2; c++) {
+            const uint32_t *poly = acc + c * N1;
+            uint32_t t[16];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                t[2 * j] = rot_minus_one(poly, abar, lane + 64 * j);
+                t[2 * j + 1] = rot_minus_one(poly, abar, lane + 64 * j + 512);
+            }
+#pragma unroll 1
+            for (int d = 0; d < Lvl1::l; d++) {
+                const cd *row = bki + (c * Lvl1::l + d) * 2 * kPolyCd;
+                cd v[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) v[j] = cd{digit1(t[2 * j], d), digit1(t[2 * j + 1], d)};
+                fft512_fwd(v, tile, lane, tw);
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    cmac(fd0[q], v[q], row[q * 64 + lane]);
+                    cmac(fd1[q], v[q], row[8 * 64 + q * 64 + lane]);
+                }
+            }
+        }
+        // every lane's rotation reads are complete (the FFTs synchronised)
+        fft512_inv(fd0, tile, lane, tw);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int n = lane + 64 * j;
+            acc[n] += trunc_torus32(fd0[j].x * (1.0 / 512));
+            acc[n + 512] += trunc_torus32(fd0[j].y * (1.0 / 512));
+        }
+        fft512_inv(fd1, tile, lane, tw);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int n = lane + 64 * j;
+            acc[N1 + n] += trunc_torus32(fd1[j].x * (1.0 / 512));
+            acc[N1 + n + 512] += trunc_torus32(fd1[j].y * (1.0 / 512));
+        }
+        __syncthreads();
+    }
+
+    if (sample_extract) {
+        uint32_t *o = out + size_t(ct) * (N1 + 1);
+#pragma unroll
+        for (int k = 0; k < N1 / 64; k++) {
+            const int n = lane + 64 * k;
+            o[n] = n == 0 ? acc[0] : 0u - acc[N1 - n];
+        }
+        if (lane == 0) o[N1] = acc[N1];
+    } else {
+        uint32_t *o = out + size_t(ct) * 2 * N1;
+#pragma unroll
+        for (int k = 0; k < 2 * N1 / 64; k++) o[lane + 64 * k] = acc[lane + 64 * k];
+    }
+}
+
+// Batched external product with one TRGSW (device layout [6][2][8][64]).
+__global__ __launch_bounds__(64) void k_extprod1(const uint32_t *__restrict__ trlwe,
+                                                 const cd *__restrict__ trgsw,
+                                                 uint32_t *__restrict__ out, int B)
+{
+    __shared__ cd tile[kFftLdsElems];
+    const int lane = threadIdx.x;
+    const int ct = blockIdx.x;
+    if (ct >= B) return;
+    const uint32_t *src = trlwe + size_t(ct) * 2 * N1;
+    const Twiddles1024 tw = make_twiddles1024(lane);
+    cd fd0[8], fd1[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) fd0[q] = fd1[q] = cd{0.0, 0.0};
+    for (int c = 0; c < 2; c++) {
+        uint32_t t[16];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            t[2 * j] = src[c * N1 + lane + 64 * j];
+            t[2 * j + 1] = src[c * N1 + lane + 64 * j + 512];
+        }
+        for (int d = 0; d < Lvl1::l; d++) {
+            cd b0[8], b1[8];
+            load_row(trgsw + (c * Lvl1::l + d) * 2 * kPolyCd, lane, b0, b1);
+            cd v[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) v[j] = cd{digit1(t[2 * j], d), digit1(t[2 * j + 1], d)};
+            fft512_fwd(v, tile, lane, tw);
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                cmac(fd0[q], v[q], b0[q]);
+                cmac(fd1[q], v[q], b1[q]);
+            }
+        }
+    }
+    uint32_t *o = out + size_t(ct) * 2 * N1;
+    fft512_inv(fd0, tile, lane, tw);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        o[lane + 64 * j] = trunc_torus32(fd0[j].x * (1.0 / 512));
+        o[lane + 64 * j + 512] = trunc_torus32(fd0[j].y * (1.0 / 512));
+    }
+    fft512_inv(fd1, tile, lane, tw);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        o[N1 + lane + 64 * j] = trunc_torus32(fd1[j].x * (1.0 / 512));
+        o[N1 + lane + 64 * j + 512] = trunc_torus32(fd1[j].y * (1.0 / 512));
+    }
+}
+
+// torus polynomial -> device-layout FD (one wave per polynomial)
+__global__ __launch_bounds__(64) void k_poly_to_devfd1(const uint32_t *__restrict__ poly,
+                                                        cd *__restrict__ fd, int npoly)
+{
+    __shared__ cd tile[kFftLdsElems];
+    const int lane = threadIdx.x;
+    const int p = blockIdx.x;
+    if (p >= npoly) return;
+    const uint32_t *a = poly + size_t(p) * N1;
+    const Twiddles1024 tw = make_twiddles1024(lane);
+    cd v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        v[j] = cd{double(int32_t(a[lane + 64 * j])), double(int32_t(a[lane + 64 * j + 512]))};
+    fft512_fwd(v, tile, lane, tw);
+#pragma unroll
+    for (int q = 0; q < 8; q++) fd[size_t(p) * kPolyCd + q * 64 + lane] = v[q];
+}
+
+// spqlios-order FD polynomials -> device layout (pure permutation)
+__global__ void k_spq_to_devfd1(const double *__restrict__ spq, cd *__restrict__ fd, int npoly)
+{
+    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= size_t(npoly) * kPolyCd) return;
+    const size_t p = gid / kPolyCd;
+    const int e = int(gid % kPolyCd);
+    const int q = e >> 6, s = e & 63;
+    const int idx = rev9(fd_index1024(s, q));
+    fd[gid] = cd{spq[p * N1 + idx], spq[p * N1 + idx + 512]};
+}
+
+// torus polynomial -> spqlios-order FD (TwistIFFT)
+__global__ __launch_bounds__(64) void k_ifft1(const uint32_t *__restrict__ poly,
+                                               double *__restrict__ fd, int B)
+{
+    __shared__ cd tile[kFftLdsElems];
+    const int lane = threadIdx.x;
+    const int p = blockIdx.x;
+    if (p >= B) return;
+    const uint32_t *a = poly + size_t(p) * N1;
+    const Twiddles1024 tw = make_twiddles1024(lane);
+    cd v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        v[j] = cd{double(int32_t(a[lane + 64 * j])), double(int32_t(a[lane + 64 * j + 512]))};
+    fft512_fwd(v, tile, lane, tw);
+    double *o = fd + size_t(p) * N1;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int idx = rev9(fd_index1024(lane, q));
+        o[idx] = v[q].x;
+        o[idx + 512] = v[q].y;
+    }
+}
+
+// spqlios-order FD -> torus polynomial (TwistFFT: 2/N scale, truncation)
+__global__ __launch_bounds__(64) void k_fft1(const double *__restrict__ fd,
+                                              uint32_t *__restrict__ poly, int B)
+{
+    __shared__ cd tile[kFftLdsElems];
+    const int lane = threadIdx.x;
+    const int p = blockIdx.x;
+    if (p >= B) return;
+    const double *f = fd + size_t(p) * N1;
+    const Twiddles1024 tw = make_twiddles1024(lane);
+    cd v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int idx = rev9(fd_index1024(lane, q));
+        v[q] = cd{f[idx], f[idx + 512]};
+    }
+    fft512_inv(v, tile, lane, tw);
+    uint32_t *o = poly + size_t(p) * N1;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        o[lane + 64 * j] = trunc_torus32(v[j].x * (1.0 / 512));
+        o[lane + 64 * j + 512] = trunc_torus32(v[j].y * (1.0 / 512));
+    }
+}
+
+__global__ void k_sample_extract1(const uint32_t *__restrict__ trlwe, uint32_t *__restrict__ tlwe,
+                                  int index, int B)
+{
+    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= size_t(B) * (N1 + 1)) return;
+    const size_t b = gid / (N1 + 1);
+    const int i = int(gid % (N1 + 1));
+    const uint32_t *a = trlwe + b * 2 * N1;
+    uint32_t v;
+    if (i == N1)
+        v = a[N1 + index];
+    else if (i <= index)
+        v = a[index - i];
+    else
+        v = 0u - a[N1 + index - i];
+    tlwe[gid] = v;
+}
+
+inline unsigned grid1d(size_t n, unsigned block) { return unsigned((n + block - 1) / block); }
+
+}  // namespace
+
+size_t bk1_device_elems(int n0) { return size_t(n0) * Lvl1::rows * 2 * kPolyCd; }
+
+hipError_t launch_bk1_from_int(const uint32_t *bk_int, cd *bk_dev, int n0, hipStream_t s)
+{
+    const int npoly = n0 * Lvl1::rows * 2;
+    hipLaunchKernelGGL(k_poly_to_devfd1, dim3(npoly), dim3(64), 0, s, bk_int, bk_dev, npoly);
+    return hipGetLastError();
+}
+
+hipError_t launch_bk1_from_fft(const double *bk_fft, cd *bk_dev, int n0, hipStream_t s)
+{
+    const int npoly = n0 * Lvl1::rows * 2;
+    hipLaunchKernelGGL(k_spq_to_devfd1, dim3(grid1d(size_t(npoly) * kPolyCd, 256)), dim3(256), 0,
+                       s, bk_fft, bk_dev, npoly);
+    return hipGetLastError();
+}
+
+hipError_t launch_blind_rotate1(const uint32_t *in, int n0, const cd *bk, const uint32_t *tv,
+                                int tv_stride, uint32_t *out, bool sample_extract, int B,
+                                hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_blind_rotate1, dim3(B), dim3(64), 0, s, in, n0, bk, tv, tv_stride, out,
+                       int(sample_extract), B);
+    return hipGetLastError();
+}
+
+hipError_t launch_trgsw1_from_fft(const double *trgswfft, cd *trgsw_dev, hipStream_t s)
+{
+    const int npoly = Lvl1::rows * 2;
+    hipLaunchKernelGGL(k_spq_to_devfd1, dim3(grid1d(size_t(npoly) * kPolyCd, 256)), dim3(256), 0,
+                       s, trgswfft, trgsw_dev, npoly);
+    return hipGetLastError();
+}
+
+hipError_t launch_extprod1(const uint32_t *trlwe, const cd *trgsw_dev, uint32_t *out, int B,
+                           hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_extprod1, dim3(B), dim3(64), 0, s, trlwe, trgsw_dev, out, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_ifft1(const uint32_t *poly, double *fd, int B, hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ifft1, dim3(B), dim3(64), 0, s, poly, fd, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_fft1(const double *fd, uint32_t *poly, int B, hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fft1, dim3(B), dim3(64), 0, s, fd, poly, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_extract1(const uint32_t *trlwe, uint32_t *tlwe, int index, int B,
+                                  hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sample_extract1, dim3(grid1d(size_t(B) * (N1 + 1), 256)), dim3(256), 0, s,
+                       trlwe, tlwe, index, B);
+    return hipGetLastError();
+}
+
+}  // namespace fhe

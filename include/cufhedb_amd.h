@@ -1,0 +1,221 @@
+/*
+ * cufhedb_amd.h -- C ABI of the MI355X (gfx950) TFHE bootstrapping library.
+ *
+ * Drop-in boundary for cuFHEDB's GPU hot path.  Every entry point names the
+ * reference interface it replaces (paths relative to the reference root):
+ *   src/cuHEDB/fft_gpu/cufft_gpu.cuh     class cufft::CuFFT_Torus<P>
+ *   src/cuHEDB/gatebootstrap_gpu.cuh     cuHEDB:: bootstrap operators
+ *   thirdparty/TFHEpp/...                CPU primitives the operators call
+ *
+ * Conventions
+ *   - All functions return an int status (FHE_OK == 0).  Nothing throws.
+ *     fhe_last_error() returns a thread-local message for the last failure.
+ *   - Ciphertexts use TFHEpp's aggregate layouts (params.hpp:31-99): a TLWE is
+ *     n+1 torus words with b last; a TRLWE is 2 polynomials of N words; a
+ *     TRGSW is (k+1)l = 2l TRLWEs.  lvl0/lvl1 words are uint32_t, lvl2 words
+ *     are uint64_t.  Batches are contiguous arrays of B such records.
+ *   - "_dev" buffers are device pointers (hipMalloc / fhe_malloc); the call is
+ *     enqueued on `stream` (a hipStream_t, NULL = the context's stream) and
+ *     returns without synchronising.  Other pointers are host memory and the
+ *     call is synchronous, like the reference's *_st entry points.
+ *   - One context owns the device keys and scratch of one GPU.  Calls that
+ *     share a context must come from one host thread at a time.
+ */
+#ifndef CUFHEDB_AMD_H
+#define CUFHEDB_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum fhe_status {
+    FHE_OK = 0,
+    FHE_ERR_ARG = 1,         /* bad size / pointer / enum argument       */
+    FHE_ERR_NOT_LOADED = 2,  /* the key this call needs was not loaded   */
+    FHE_ERR_DEVICE = 3,      /* HIP runtime error (message has details)   */
+    FHE_ERR_ALLOC = 4,       /* device or host allocation failed          */
+    FHE_ERR_UNSUPPORTED = 5  /* option not available in this build        */
+};
+
+/* Runtime parameter set.  Ring/gadget/key-switch constants are fixed to the
+ * reference's params/128bit.hpp (lvl1 N=1024,l=3,Bgbit=6; lvl2 N=2048,l=4,
+ * Bgbit=9; KS10/20 t=2,basebit=10; KS21 t=10,basebit=3); the lvl0 dimension
+ * and the key / noise distributions are runtime values. */
+typedef struct fhe_params {
+    uint32_t n0;      /* lvl0 LWE dimension (reference: 672)                  */
+    uint32_t key0_hw; /* lvl0 key Hamming weight, 0 = uniform binary (512)   */
+    uint32_t key1_hw; /* lvl1 key Hamming weight, 0 = uniform binary (512)   */
+    uint32_t key2_hw; /* lvl2 key Hamming weight, 0 = uniform binary (1024)  */
+    double alpha0;    /* lvl0 noise stdev, also KS10 / KS20 keys (2^-16)     */
+    double alpha1;    /* lvl1 noise stdev: BK lvl01, KS21 (2^-25)            */
+    double alpha2;    /* lvl2 noise stdev: BK lvl02 (2^-52)                  */
+} fhe_params;
+
+enum fhe_preset {
+    FHE_PRESET_REF672 = 0, /* thirdparty/TFHEpp/include/params/128bit.hpp + key.cpp:8-17 */
+    FHE_PRESET_CFG636 = 1  /* n0 = 636, alpha0 = 2^-15, uniform binary keys          */
+};
+int fhe_params_preset(int preset, fhe_params *out);
+
+const char *fhe_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side key generation and encryption (deterministic from a 64-bit seed).
+ * Replace TFHEpp's unseeded lweKey() (src/key.cpp:8-17), bkgen
+ * (src/cloudkey.cpp:7-21), ikskgen (src/cloudkey.cpp:54-70) and
+ * tlweSymEncrypt (src/tlwe.cpp:19-31); same distributions, seeded RNG
+ * (Philox4x32-10) so that tests and benches are reproducible.            */
+int fhe_keygen_secret(const fhe_params *p, uint64_t seed, uint32_t *s0 /* n0 */,
+                      uint32_t *s1 /* 1024 */, uint64_t *s2 /* 2048 */);
+/* Integer bootstrapping keys: TFHEpp BootstrappingKey<lvl01param/lvl02param>
+ * layout [n0][2l][2][N] (params.hpp:67-69). */
+int fhe_keygen_bk_lvl01(const fhe_params *p, uint64_t seed, const uint32_t *s0, const uint32_t *s1,
+                        uint32_t *bk);
+int fhe_keygen_bk_lvl02(const fhe_params *p, uint64_t seed, const uint32_t *s0, const uint64_t *s2,
+                        uint64_t *bk);
+/* Key-switching keys, TFHEpp KeySwitchingKey layout [dn][t][2^basebit-1][tn+1]:
+ *   kind 10: lvl1 (dn=1024) -> lvl0 (tn=n0), s_dom = s1 (uint32_t*)
+ *   kind 20: lvl2 (dn=2048) -> lvl0 (tn=n0), s_dom = s2 (uint64_t*)
+ *   kind 21: lvl2 (dn=2048) -> lvl1 (tn=1024), s_dom = s2 (uint64_t*) */
+int fhe_keygen_ksk(const fhe_params *p, int kind, uint64_t seed, const void *s_dom,
+                   const uint32_t *s_tgt, uint32_t *ksk);
+size_t fhe_ksk_words(const fhe_params *p, int kind);
+/* TLWE encryptions of B messages (torus values m[i]) under `key` (n words),
+ * noise stdev alpha; ciphertext i draws from RNG stream (seed, index0 + i). */
+int fhe_tlwe_encrypt_u32(uint64_t seed, uint64_t index0, const uint32_t *key, uint32_t n,
+                         const uint32_t *m, double alpha, uint32_t *out, int B);
+int fhe_tlwe_encrypt_u64(uint64_t seed, uint64_t index0, const uint64_t *key, uint32_t n,
+                         const uint64_t *m, double alpha, uint64_t *out, int B);
+/* TRLWE encryption of B polynomials (trlweSymEncrypt, src/trlwe.cpp:30-37). */
+int fhe_trlwe_encrypt_u32(uint64_t seed, uint64_t index0, const uint32_t *key /* N */,
+                          const uint32_t *m /* B x N */, double alpha, uint32_t *out, int B);
+int fhe_trlwe_encrypt_u64(uint64_t seed, uint64_t index0, const uint64_t *key,
+                          const uint64_t *m, double alpha, uint64_t *out, int B);
+/* TRGSW of the constant polynomial m (trgswSymEncrypt, src/trgsw.cpp:206-224). */
+int fhe_trgsw_encrypt_u32(uint64_t seed, uint64_t index, const uint32_t *key, int32_t m,
+                          double alpha, uint32_t *out /* 2l x 2 x N */);
+int fhe_trgsw_encrypt_u64(uint64_t seed, uint64_t index, const uint64_t *key, int64_t m,
+                          double alpha, uint64_t *out);
+
+/* ------------------------------------------------------------------------ */
+/* Device context.  Replaces cufft::CuFFT_Torus<P>::CuFFT_Torus
+ * (src/cuHEDB/fft_gpu/cufft_gpu.cuh:135-163) and the global singletons
+ * cufftlvl1/cufftlvl2 (src/cuHEDB/gatebootstrap_gpu.cuh:24-25). */
+typedef struct fhe_ctx fhe_ctx;
+int fhe_ctx_create(const fhe_params *p, int device, fhe_ctx **out);
+int fhe_ctx_destroy(fhe_ctx *ctx);
+int fhe_ctx_sync(fhe_ctx *ctx);
+void *fhe_ctx_stream(fhe_ctx *ctx);
+
+/* Bootstrapping-key upload (CuFFT_Torus::LoadBK, cufft_gpu.cuh:175-184).
+ * format FHE_BK_INT:   integer TRGSW key [n0][2l][2][N] (lvl1 uint32_t,
+ *                      lvl2 uint64_t); the FFT to the device FD layout runs
+ *                      on the GPU.
+ * format FHE_BK_FFT:   TFHEpp BootstrappingKeyFFT [n0][2l][2][N] doubles in
+ *                      spqlios order (what LoadBK takes); permuted on load. */
+enum fhe_bk_format { FHE_BK_INT = 0, FHE_BK_FFT = 1 };
+int fhe_load_bk(fhe_ctx *ctx, int level /* 1 or 2 */, const void *bk_host, int format);
+/* Key-switching key upload (host TFHEpp layout, kind 10/20/21).  The
+ * reference keeps these on the host and switches on the CPU. */
+int fhe_load_ksk(fhe_ctx *ctx, int kind, const uint32_t *ksk_host);
+
+/* PBS kernel variant for the lvl1 blind rotation. */
+enum fhe_pbs_variant {
+    FHE_PBS_FP64 = 0 /* FP64 negacyclic FFT, LDS-resident (plain method) */
+};
+int fhe_set_pbs_variant(fhe_ctx *ctx, int variant);
+
+/* ---- batched hot path on device buffers -------------------------------- */
+/* Programmable bootstrap lvl0 -> lvl1 = BlindRotate + SampleExtractIndex(0)
+ * (TFHEpp GateBootstrappingTLWE2TLWEFFT<lvl01param>,
+ * thirdparty/TFHEpp/src/gatebootstrapping.cpp:14-23; GPU kernel
+ * GateBootstrappingTLWE2TLWEFFT<Lvl1>, src/cuHEDB/fft_gpu/BlindRotate_gpu.cuh:401-531).
+ *   in_dev : B x (n0+1) uint32_t lvl0 TLWEs
+ *   tv_dev : test vectors, N uint32_t each; tv_stride = 0 -> one test vector
+ *            for the whole batch, tv_stride = N -> one per ciphertext
+ *   out_dev: B x 1025 uint32_t lvl1 TLWEs */
+int fhe_pbs_lvl01_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint32_t *tv_dev, int tv_stride,
+                      uint32_t *out_dev, int B, void *stream);
+/* Blind rotation only (acc, B x 2 x N); BlindRotate, gatebootstrapping.hpp:15-43 */
+int fhe_blind_rotate_lvl01_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint32_t *tv_dev,
+                               int tv_stride, uint32_t *acc_dev, int B, void *stream);
+/* lvl0 -> lvl2 PBS (GateBootstrappingTLWE2TLWEFFT<lvl02param>;
+ * BlindRotate_gpu.cuh:534-670), uint64_t test vectors / outputs. */
+int fhe_pbs_lvl02_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint64_t *tv_dev, int tv_stride,
+                      uint64_t *out_dev, int B, void *stream);
+/* SampleExtractIndex (thirdparty/TFHEpp/src/trlwe.cpp:109-118;
+ * BlindRotate_gpu.cuh:386-397,703-718): B TRLWEs -> B TLWEs. */
+int fhe_sample_extract_lvl1_dev(fhe_ctx *ctx, const uint32_t *trlwe_dev, uint32_t *tlwe_dev,
+                                int index, int B, void *stream);
+int fhe_sample_extract_lvl2_dev(fhe_ctx *ctx, const uint64_t *trlwe_dev, uint64_t *tlwe_dev,
+                                int index, int B, void *stream);
+/* IdentityKeySwitch (thirdparty/TFHEpp/src/keyswitch.cpp:12-48); the
+ * reference runs it on the host before every bootstrap
+ * (gatebootstrap_gpu.cuh:42,59,76,90,101,113,133,149,159,167).
+ *   kind 10: in B x 1025 uint32_t -> out B x (n0+1) uint32_t
+ *   kind 20: in B x 2049 uint64_t -> out B x (n0+1) uint32_t
+ *   kind 21: in B x 2049 uint64_t -> out B x 1025 uint32_t */
+int fhe_iks_dev(fhe_ctx *ctx, int kind, const void *in_dev, uint32_t *out_dev, int B, void *stream);
+/* External product TRGSW (x) TRLWE for a batch of TRLWEs and one TRGSW given
+ * in TFHEpp TRGSWFFT layout (spqlios order) on the device
+ * (trgswfftExternalProduct, TFHEpp/src/trgsw.cpp:78-103; CuFFT_Torus::
+ * ExternalProduct_st, cufft_gpu.cuh:228-248). */
+int fhe_external_product_lvl1_dev(fhe_ctx *ctx, const uint32_t *trlwe_dev,
+                                  const double *trgswfft_dev, uint32_t *out_dev, int B,
+                                  void *stream);
+int fhe_external_product_lvl2_dev(fhe_ctx *ctx, const uint64_t *trlwe_dev,
+                                  const double *trgswfft_dev, uint64_t *out_dev, int B,
+                                  void *stream);
+/* Twisted negacyclic transforms in TFHEpp's FD layout (CuFFT_Torus::ifft_st /
+ * fft_st, cufft_gpu.cuh:187-212; TwistIFFT / TwistFFT, mulfft.hpp:58-121):
+ * B polynomials per call. */
+int fhe_ifft_lvl1_dev(fhe_ctx *ctx, const uint32_t *poly_dev, double *fd_dev, int B, void *stream);
+int fhe_fft_lvl1_dev(fhe_ctx *ctx, const double *fd_dev, uint32_t *poly_dev, int B, void *stream);
+int fhe_ifft_lvl2_dev(fhe_ctx *ctx, const uint64_t *poly_dev, double *fd_dev, int B, void *stream);
+int fhe_fft_lvl2_dev(fhe_ctx *ctx, const double *fd_dev, uint64_t *poly_dev, int B, void *stream);
+
+/* ---- bootstrap operators (cuHEDB::, gatebootstrap_gpu.cuh:31-170) --------- */
+enum fhe_gate {
+    FHE_GATE_NAND = 0, /* TFHEpp HomNAND, gate.cpp:60-65: -a-b+mu          */
+    FHE_GATE_AND = 1,  /* cuHEDB::HomAND, gatebootstrap_gpu.cuh:123-137     */
+    FHE_GATE_OR = 2,   /* cuHEDB::HomOR, gatebootstrap_gpu.cuh:139-153      */
+    FHE_GATE_NOR = 3,  /* TFHEpp HomNOR, gate.cpp:73-78                    */
+    FHE_GATE_XNOR = 4  /* TFHEpp HomXNOR, gate.cpp:86-91                   */
+};
+/* Two-input gate on lvl1 TLWEs: linear combination, IKS lvl10, PBS.
+ * result_type: 1 = LOGIC (+-mu), 0 = ARITHMETIC (0 / 2mu), as in
+ * HEDB/utils/types.h:4-11; NAND/NOR/XNOR are always LOGIC. */
+int fhe_gate_lvl1_dev(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint32_t *b_dev,
+                      uint32_t *out_dev, int result_type, int B, void *stream);
+/* cuHEDB::MSBGateBootstrapping (lvl1 -> lvl1, gatebootstrap_gpu.cuh:31-46) */
+int fhe_msb_gate_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_dev,
+                          int result_type, int B, void *stream);
+/* cuHEDB::IdeGateBootstrapping (lvl1 -> lvl1, gatebootstrap_gpu.cuh:82-92) */
+int fhe_ide_gate_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_dev,
+                          uint32_t scale_bits, int B, void *stream);
+
+/* ---- host-pointer convenience forms (synchronous, like the *_st calls) -- */
+int fhe_pbs_lvl01(fhe_ctx *ctx, const uint32_t *in, const uint32_t *tv, int tv_stride,
+                  uint32_t *out, int B);
+int fhe_iks(fhe_ctx *ctx, int kind, const void *in, uint32_t *out, int B);
+
+/* ---- device memory and timing helpers ---------------------------------- */
+int fhe_malloc(void **ptr, size_t bytes);
+int fhe_free(void *ptr);
+int fhe_memcpy_h2d(void *dst_dev, const void *src, size_t bytes);
+int fhe_memcpy_d2h(void *dst, const void *src_dev, size_t bytes);
+int fhe_memset_dev(void *dst_dev, int value, size_t bytes);
+int fhe_device_sync(void);
+/* Elapsed milliseconds of the work enqueued on `stream` between
+ * fhe_timer_start and fhe_timer_stop (HIP events on that stream). */
+int fhe_timer_start(fhe_ctx *ctx, void *stream);
+int fhe_timer_stop(fhe_ctx *ctx, void *stream, float *ms);
+int fhe_device_name(char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,224 @@
+// ref_harness.cpp -- extern "C" entry points into the reference TFHEpp CPU
+// library, compiled from the reference's own sources (see oracle/Makefile).
+//
+// TEST INFRASTRUCTURE ONLY: used to pin the oracle (tfhe_oracle.c) and to
+// write golden vectors under tests/golden/.  Keys come from the caller (the
+// reference's own key generator cannot be seeded: TFHEpp/include/utils.hpp:17-22),
+// so every function takes raw arrays laid out as TFHEpp's std::array
+// aggregates (TFHEpp/include/params.hpp:31-99) and reinterprets them.
+//
+// TFHEpp is compiled with its default parameter set params/128bit.hpp, i.e.
+// lvl0 n = 672; the n = 636 set is checked through the oracle only.
+#include <cstring>
+#include <memory>
+#include <tfhe++.hpp>
+
+#include "HEDB/comparison/tfhepp_utils.h"
+
+using namespace TFHEpp;
+
+namespace {
+template <class A>
+const A &as(const void *p)
+{
+    return *reinterpret_cast<const A *>(p);
+}
+}  // namespace
+
+extern "C" {
+
+int ref_lvl0_n() { return lvl0param::n; }
+
+void ref_twist_ifft_lvl1(double *fd, const uint32_t *a)
+{
+    PolynomialInFD<lvl1param> r;
+    TwistIFFT<lvl1param>(r, as<Polynomial<lvl1param>>(a));
+    std::memcpy(fd, r.data(), sizeof(r));
+}
+
+void ref_twist_fft_lvl1(uint32_t *res, const double *fd)
+{
+    Polynomial<lvl1param> r;
+    TwistFFT<lvl1param>(r, as<PolynomialInFD<lvl1param>>(fd));
+    std::memcpy(res, r.data(), sizeof(r));
+}
+
+void ref_twist_ifft_lvl2(double *fd, const uint64_t *a)
+{
+    PolynomialInFD<lvl2param> r;
+    TwistIFFT<lvl2param>(r, as<Polynomial<lvl2param>>(a));
+    std::memcpy(fd, r.data(), sizeof(r));
+}
+
+void ref_twist_fft_lvl2(uint64_t *res, const double *fd)
+{
+    Polynomial<lvl2param> r;
+    TwistFFT<lvl2param>(r, as<PolynomialInFD<lvl2param>>(fd));
+    std::memcpy(res, r.data(), sizeof(r));
+}
+
+void ref_polymul_lvl1(uint32_t *res, const uint32_t *a, const uint32_t *b)
+{
+    Polynomial<lvl1param> r;
+    PolyMul<lvl1param>(r, as<Polynomial<lvl1param>>(a), as<Polynomial<lvl1param>>(b));
+    std::memcpy(res, r.data(), sizeof(r));
+}
+
+void ref_decompose_lvl1(uint32_t *dec, const uint32_t *poly, int digit)
+{
+    DecomposedPolynomial<lvl1param> r;
+    DecompositionPolynomial<lvl1param>(r, as<Polynomial<lvl1param>>(poly), digit);
+    std::memcpy(dec, r.data(), sizeof(r));
+}
+
+void ref_decompose_lvl2(uint64_t *dec, const uint64_t *poly, int digit)
+{
+    DecomposedPolynomial<lvl2param> r;
+    DecompositionPolynomial<lvl2param>(r, as<Polynomial<lvl2param>>(poly), digit);
+    std::memcpy(dec, r.data(), sizeof(r));
+}
+
+void ref_trgsw_to_fft_lvl1(double *out, const uint32_t *trgsw)
+{
+    auto r = std::make_unique<TRGSWFFT<lvl1param>>();
+    *r = ApplyFFT2trgsw<lvl1param>(as<TRGSW<lvl1param>>(trgsw));
+    std::memcpy(out, r->data(), sizeof(*r));
+}
+
+void ref_trgsw_to_fft_lvl2(double *out, const uint64_t *trgsw)
+{
+    auto r = std::make_unique<TRGSWFFT<lvl2param>>();
+    *r = ApplyFFT2trgsw<lvl2param>(as<TRGSW<lvl2param>>(trgsw));
+    std::memcpy(out, r->data(), sizeof(*r));
+}
+
+void ref_external_product_lvl1(uint32_t *res, const uint32_t *trlwe, const double *trgswfft)
+{
+    TRLWE<lvl1param> r;
+    trgswfftExternalProduct<lvl1param>(r, as<TRLWE<lvl1param>>(trlwe),
+                                       as<TRGSWFFT<lvl1param>>(trgswfft));
+    std::memcpy(res, r.data(), sizeof(r));
+}
+
+void ref_external_product_lvl2(uint64_t *res, const uint64_t *trlwe, const double *trgswfft)
+{
+    TRLWE<lvl2param> r;
+    trgswfftExternalProduct<lvl2param>(r, as<TRLWE<lvl2param>>(trlwe),
+                                       as<TRGSWFFT<lvl2param>>(trgswfft));
+    std::memcpy(res, r.data(), sizeof(r));
+}
+
+void ref_blind_rotate_lvl01(uint32_t *acc, const uint32_t *tlwe0, const double *bkfft,
+                            const uint32_t *tv)
+{
+    TRLWE<lvl1param> r;
+    BlindRotate<lvl01param>(r, as<TLWE<lvl0param>>(tlwe0),
+                            as<BootstrappingKeyFFT<lvl01param>>(bkfft),
+                            as<Polynomial<lvl1param>>(tv));
+    std::memcpy(acc, r.data(), sizeof(r));
+}
+
+void ref_pbs_lvl01(uint32_t *out, const uint32_t *tlwe0, const double *bkfft, const uint32_t *tv)
+{
+    TLWE<lvl1param> r;
+    GateBootstrappingTLWE2TLWEFFT<lvl01param>(r, as<TLWE<lvl0param>>(tlwe0),
+                                              as<BootstrappingKeyFFT<lvl01param>>(bkfft),
+                                              as<Polynomial<lvl1param>>(tv));
+    std::memcpy(out, r.data(), sizeof(r));
+}
+
+void ref_pbs_lvl02(uint64_t *out, const uint32_t *tlwe0, const double *bkfft, const uint64_t *tv)
+{
+    TLWE<lvl2param> r;
+    GateBootstrappingTLWE2TLWEFFT<lvl02param>(r, as<TLWE<lvl0param>>(tlwe0),
+                                              as<BootstrappingKeyFFT<lvl02param>>(bkfft),
+                                              as<Polynomial<lvl2param>>(tv));
+    std::memcpy(out, r.data(), sizeof(r));
+}
+
+void ref_sample_extract_lvl1(uint32_t *tlwe, const uint32_t *trlwe, int index)
+{
+    TLWE<lvl1param> r;
+    SampleExtractIndex<lvl1param>(r, as<TRLWE<lvl1param>>(trlwe), index);
+    std::memcpy(tlwe, r.data(), sizeof(r));
+}
+
+void ref_sample_extract_lvl2(uint64_t *tlwe, const uint64_t *trlwe, int index)
+{
+    TLWE<lvl2param> r;
+    SampleExtractIndex<lvl2param>(r, as<TRLWE<lvl2param>>(trlwe), index);
+    std::memcpy(tlwe, r.data(), sizeof(r));
+}
+
+void ref_iks_lvl10(uint32_t *out, const uint32_t *in, const uint32_t *ksk)
+{
+    TLWE<lvl0param> r;
+    IdentityKeySwitch<lvl10param>(r, as<TLWE<lvl1param>>(in),
+                                  as<KeySwitchingKey<lvl10param>>(ksk));
+    std::memcpy(out, r.data(), sizeof(r));
+}
+
+void ref_iks_lvl20(uint32_t *out, const uint64_t *in, const uint32_t *ksk)
+{
+    TLWE<lvl0param> r;
+    IdentityKeySwitch<lvl20param>(r, as<TLWE<lvl2param>>(in),
+                                  as<KeySwitchingKey<lvl20param>>(ksk));
+    std::memcpy(out, r.data(), sizeof(r));
+}
+
+void ref_iks_lvl21(uint32_t *out, const uint64_t *in, const uint32_t *ksk)
+{
+    TLWE<lvl1param> r;
+    IdentityKeySwitch<lvl21param>(r, as<TLWE<lvl2param>>(in),
+                                  as<KeySwitchingKey<lvl21param>>(ksk));
+    std::memcpy(out, r.data(), sizeof(r));
+}
+
+// EvalKey adopts the caller's key arrays for the duration of one call and
+// releases them before it is destroyed (it never owns them).
+namespace {
+struct BorrowedKey {
+    EvalKey ek;
+    BorrowedKey(const double *bkfft, const uint32_t *ksk10)
+    {
+        ek.bkfftlvl01.reset(reinterpret_cast<BootstrappingKeyFFT<lvl01param> *>(
+            const_cast<double *>(bkfft)));
+        ek.iksklvl10.reset(reinterpret_cast<KeySwitchingKey<lvl10param> *>(
+            const_cast<uint32_t *>(ksk10)));
+    }
+    ~BorrowedKey()
+    {
+        ek.bkfftlvl01.release();
+        ek.iksklvl10.release();
+    }
+};
+}  // namespace
+
+void ref_hom_nand_lvl1(uint32_t *out, const uint32_t *ca, const uint32_t *cb,
+                       const double *bkfft, const uint32_t *ksk10)
+{
+    BorrowedKey k(bkfft, ksk10);
+    TLWE<lvl1param> r;
+    HomNAND<lvl1param>(r, as<TLWE<lvl1param>>(ca), as<TLWE<lvl1param>>(cb), k.ek);
+    std::memcpy(out, r.data(), sizeof(r));
+}
+
+void ref_msb_gate_lvl1(uint32_t *out, const uint32_t *in, const double *bkfft,
+                       const uint32_t *ksk10, int result_type)
+{
+    BorrowedKey k(bkfft, ksk10);
+    TLWE<lvl1param> r;
+    MSBGateBootstrapping(r, as<TLWE<lvl1param>>(in), k.ek, result_type != 0);
+    std::memcpy(out, r.data(), sizeof(r));
+}
+
+void ref_ide_gate_lvl1(uint32_t *out, const uint32_t *in, uint32_t scale_bits,
+                       const double *bkfft, const uint32_t *ksk10)
+{
+    BorrowedKey k(bkfft, ksk10);
+    TLWE<lvl1param> r;
+    IdeGateBootstrapping(r, as<TLWE<lvl1param>>(in), scale_bits, k.ek);
+    std::memcpy(out, r.data(), sizeof(r));
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+"""ctypes bindings to the checker libraries (test infrastructure only).
+
+* ``orc``: oracle/liboracle.so, the C restatement of the reference path.
+* ``ref``: oracle/_ref/libtfhepp_ref.so, the reference's own TFHEpp compiled
+  from its sources (present only where the reference sources were available
+  at build time; tests that need it skip otherwise).
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libtfhepp_ref.so")
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_u32 = ctypes.c_uint32
+
+
+def P(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+_orc = None
+_ref = None
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} missing: run `make -C oracle`")
+        L = ctypes.CDLL(ORACLE_SO)
+        L.orc_phase_u32.restype = ctypes.c_uint32
+        L.orc_phase_u64.restype = ctypes.c_uint64
+        L.orc_fd_rev.restype = _i
+        _orc = L
+    return _orc
+
+
+def ref():
+    """The compiled reference, or None."""
+    global _ref
+    if _ref is None and os.path.exists(REF_SO):
+        _ref = ctypes.CDLL(REF_SO)
+    return _ref
+
+
+def ref_n0():
+    r = ref()
+    return r.ref_lvl0_n() if r is not None else None
+
+
+# ---------------------------------------------------------------- wrappers
+def ifft32(a, N=1024):
+    a = np.ascontiguousarray(a, np.uint32)
+    out = np.zeros(N)
+    orc().orc_ifft_torus32(P(out), P(a), N)
+    return out
+
+
+def fft32(fd, N=1024):
+    fd = np.ascontiguousarray(fd, np.float64)
+    out = np.zeros(N, np.uint32)
+    orc().orc_fft_torus32(P(out), P(fd), N)
+    return out
+
+
+def ifft64(a, N=2048):
+    a = np.ascontiguousarray(a, np.uint64)
+    out = np.zeros(N)
+    orc().orc_ifft_torus64(P(out), P(a), N)
+    return out
+
+
+def fft64(fd, N=2048):
+    fd = np.ascontiguousarray(fd, np.float64)
+    out = np.zeros(N, np.uint64)
+    orc().orc_fft_torus64(P(out), P(fd), N)
+    return out
+
+
+def naive32(a, b):
+    a = np.ascontiguousarray(a, np.uint32)
+    b = np.ascontiguousarray(b, np.uint32)
+    out = np.zeros_like(b)
+    orc().orc_negacyclic_naive_u32(P(out), P(a), P(b), b.shape[0])
+    return out
+
+
+def naive64(a, b):
+    a = np.ascontiguousarray(a, np.uint64)
+    b = np.ascontiguousarray(b, np.uint64)
+    out = np.zeros_like(b)
+    orc().orc_negacyclic_naive_u64(P(out), P(a), P(b), b.shape[0])
+    return out
+
+
+def decompose32(poly, digit, l=3, Bgbit=6):
+    poly = np.ascontiguousarray(poly, np.uint32)
+    out = np.zeros_like(poly)
+    orc().orc_decompose_u32(P(out), P(poly), digit, poly.shape[0], l, Bgbit)
+    return out
+
+
+def decompose64(poly, digit, l=4, Bgbit=9):
+    poly = np.ascontiguousarray(poly, np.uint64)
+    out = np.zeros_like(poly)
+    orc().orc_decompose_u64(P(out), P(poly), digit, poly.shape[0], l, Bgbit)
+    return out
+
+
+def mul_xai32(poly, a, minus_one=False):
+    poly = np.ascontiguousarray(poly, np.uint32)
+    out = np.zeros_like(poly)
+    f = orc().orc_mul_xai_minus_one_u32 if minus_one else orc().orc_mul_xai_u32
+    f(P(out), P(poly), _u32(a), poly.shape[0])
+    return out
+
+
+def trgsw_to_fft32(trgsw, l=3):
+    trgsw = np.ascontiguousarray(trgsw, np.uint32)
+    N = trgsw.shape[-1]
+    out = np.zeros(trgsw.shape, np.float64)
+    orc().orc_trgsw_to_fft_u32(P(out), P(trgsw), N, l)
+    return out
+
+
+def bk_to_fft32(bk):
+    """[n0][6][2][N] uint32 -> FD doubles, same layout (ApplyFFT2trgsw per entry)."""
+    out = np.zeros(bk.shape, np.float64)
+    for i in range(bk.shape[0]):
+        out[i] = trgsw_to_fft32(bk[i])
+    return out
+
+
+def external_product32(trlwe, trgswfft, l=3, Bgbit=6):
+    trlwe = np.ascontiguousarray(trlwe, np.uint32)
+    out = np.zeros_like(trlwe)
+    orc().orc_external_product_u32(P(out), P(trlwe), P(np.ascontiguousarray(trgswfft)),
+                                   trlwe.shape[-1], l, Bgbit)
+    return out
+
+
+def blind_rotate32(tlwe0, bkfft, tv, nbit=10, l=3, Bgbit=6):
+    tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)
+    N = tv.shape[0]
+    out = np.zeros((2, N), np.uint32)
+    orc().orc_blind_rotate_u32(P(out), P(tlwe0), tlwe0.shape[0] - 1, P(bkfft),
+                               P(np.ascontiguousarray(tv, np.uint32)), N, nbit, l, Bgbit)
+    return out
+
+
+def pbs32(tlwe0, bkfft, tv, nbit=10, l=3, Bgbit=6):
+    tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)
+    N = tv.shape[0]
+    out = np.zeros(N + 1, np.uint32)
+    orc().orc_pbs_u32(P(out), P(tlwe0), tlwe0.shape[0] - 1, P(bkfft),
+                      P(np.ascontiguousarray(tv, np.uint32)), N, nbit, l, Bgbit)
+    return out
+
+
+def sample_extract32(trlwe, index=0):
+    trlwe = np.ascontiguousarray(trlwe, np.uint32)
+    N = trlwe.shape[-1]
+    out = np.zeros(N + 1, np.uint32)
+    orc().orc_sample_extract_u32(P(out), P(trlwe), index, N)
+    return out
+
+
+def iks32(tlwe, ksk, tn, t=2, basebit=10):
+    tlwe = np.ascontiguousarray(tlwe, np.uint32)
+    out = np.zeros(tn + 1, np.uint32)
+    orc().orc_iks_32to32(P(out), P(tlwe), P(ksk), tlwe.shape[0] - 1, tn, t, basebit)
+    return out
+
+
+def iks64(tlwe, ksk, tn, t, basebit):
+    tlwe = np.ascontiguousarray(tlwe, np.uint64)
+    out = np.zeros(tn + 1, np.uint32)
+    orc().orc_iks_64to32(P(out), P(tlwe), P(ksk), tlwe.shape[0] - 1, tn, t, basebit)
+    return out
+
+
+def trlwe_phase32(c, key):
+    c = np.ascontiguousarray(c, np.uint32).reshape(2, -1)
+    out = np.zeros(c.shape[1], np.uint32)
+    orc().orc_trlwe_phase_u32(P(out), P(c), P(np.ascontiguousarray(key, np.uint32)), c.shape[1])
+    return out
+
+
+def hom_nand(ca, cb, ksk10, n0, bkfft):
+    out = np.zeros(1025, np.uint32)
+    orc().orc_hom_nand_lvl1(P(out), P(np.ascontiguousarray(ca, np.uint32)),
+                            P(np.ascontiguousarray(cb, np.uint32)), P(ksk10), n0, P(bkfft))
+    return out
+
+
+def tdiff32(a, b):
+    """signed difference a - b on the 32-bit torus (int64 array)."""
+    d = (np.asarray(a, np.uint32).astype(np.int64) - np.asarray(b, np.uint32).astype(np.int64))
+    return (d + (1 << 31)) % (1 << 32) - (1 << 31)
+
+
+def tdiff64(a, b):
+    return (np.asarray(a, np.uint64) - np.asarray(b, np.uint64)).view(np.int64)

@@ -1,0 +1,101 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+TESTS = os.path.dirname(os.path.abspath(__file__))
+if TESTS not in sys.path:
+    sys.path.insert(0, TESTS)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP library)")
+    config.addinivalue_line("markers", "slow: long CPU check (reference cross-check at full size)")
+
+
+import cufhedb_amd as F  # noqa: E402
+
+
+class Keys:
+    """Seeded key material for one parameter preset (cached per session)."""
+
+    def __init__(self, which, seed):
+        self.params = F.preset(which)
+        self.n0 = self.params.n0
+        self.seed = seed
+        self.sk = F.SecretKeys(self.params, seed)
+        self.bk = F.keygen_bk_lvl01(self.sk, seed + 1)
+        self._bkfft = None
+        self._ksk10 = None
+
+    @property
+    def bkfft(self):
+        """FD key in TFHEpp BootstrappingKeyFFT layout (oracle ApplyFFT2trgsw)."""
+        if self._bkfft is None:
+            import _oracle as O
+            self._bkfft = O.bk_to_fft32(self.bk)
+        return self._bkfft
+
+    @property
+    def ksk10(self):
+        if self._ksk10 is None:
+            self._ksk10 = F.keygen_ksk(self.sk, 10, self.seed + 2)
+        return self._ksk10
+
+    def enc_lvl0(self, m, index0=0, seed=None):
+        return F.tlwe_encrypt_u32(self.seed + 100 if seed is None else seed, index0, self.sk.s0,
+                                  m, self.params.alpha0)
+
+    def enc_lvl1(self, m, index0=0, seed=None):
+        return F.tlwe_encrypt_u32(self.seed + 200 if seed is None else seed, index0, self.sk.s1,
+                                  m, self.params.alpha1)
+
+
+_KEYS = {}
+
+
+def keys_for(which):
+    if which not in _KEYS:
+        _KEYS[which] = Keys(which, {F.PRESET_REF672: 672001, F.PRESET_CFG636: 636001}[which])
+    return _KEYS[which]
+
+
+@pytest.fixture(scope="session")
+def keys672():
+    return keys_for(F.PRESET_REF672)
+
+
+@pytest.fixture(scope="session")
+def keys636():
+    return keys_for(F.PRESET_CFG636)
+
+
+_CTX = {}
+
+
+def ctx_for(which):
+    if which not in _CTX:
+        k = keys_for(which)
+        ctx = F.Context(k.params)
+        ctx.load_bk(1, k.bk, F.BK_INT)
+        _CTX[which] = ctx
+    return _CTX[which]
+
+
+@pytest.fixture(scope="session")
+def ctx672():
+    return ctx_for(F.PRESET_REF672)
+
+
+@pytest.fixture(scope="session")
+def ctx636():
+    return ctx_for(F.PRESET_CFG636)
+
+
+@pytest.fixture
+def rng():
+    return np.random.default_rng(20261202)
