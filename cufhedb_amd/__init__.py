@@ -144,7 +144,8 @@ def lib():
 
 
 def _ptr(a):
-    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+    """Pointer argument that keeps `a` alive for the duration of the call."""
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 
 
 def check(rc):

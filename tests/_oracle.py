@@ -21,7 +21,8 @@ _u32 = ctypes.c_uint32
 
 
 def P(a):
-    return ctypes.c_void_p(a.ctypes.data)
+    """Pointer argument that keeps `a` alive for the duration of the call."""
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
 _orc = None

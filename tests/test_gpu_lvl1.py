@@ -8,7 +8,10 @@ unit, test/Arc_Query/fft_res.txt):
   * external product vs oracle: <= 2 torus units per coefficient
   * sample extract: bit-exact
   * PBS / blind rotation: decrypted plaintexts bit-exact vs the LUT, output
-    phase within 2^-10 of the oracle's phase on the same keys and inputs.
+    phase within 2^-6 of the oracle's phase on the same keys and inputs (two
+    FP64 implementations accumulate independent noise after their first low-digit
+    divergence; measured gap std 2^22.8, noise std 2^22.3 -- see
+    tests/test_oracle_golden.py and tests/test_gpu_noise.py).
 """
 
 import numpy as np
@@ -21,7 +24,7 @@ pytestmark = pytest.mark.gpu
 
 N = 1024
 DELTA8 = 1 << 28  # Z_8 with one padding bit
-PHASE_TOL = 1 << 22  # 2^-10 of the torus (1/32 of the Z_8 decoding half-box)
+PHASE_TOL = 1 << 26  # 2^-6 of the torus: ~9 sigma of the measured FP64-vs-FP64 phase gap
 
 
 def z8_testvector(lut):
