@@ -19,8 +19,8 @@ HDRS := $(wildcard $(SRC)/*.hpp) include/cufhedb_amd.h
 OBJS := $(patsubst $(SRC)/%.hip,$(OBJ)/%.hip.o,$(HIP_SRCS)) \
         $(patsubst $(SRC)/%.cpp,$(OBJ)/%.cpp.o,$(CPP_SRCS))
 
-.PHONY: all lib oracle ref clean
-all: lib oracle
+.PHONY: all lib oracle ref dropin clean
+all: lib oracle dropin
 
 lib: $(LIB)
 
@@ -34,6 +34,13 @@ $(OBJ)/%.cpp.o: $(SRC)/%.cpp $(HDRS)
 
 $(LIB): $(OBJS)
 	$(HIPCC) --offload-arch=$(ARCH) -shared -fPIC -o $@ $(OBJS) -lpthread
+
+# C++ drop-in usage example / test program (run by tests/test_gpu_dropin.py)
+dropin: $(OBJ)/dropin_demo
+
+$(OBJ)/dropin_demo: tests/cpp/dropin_demo.cpp include/cufhedb_amd/cuhedb.hpp include/cufhedb_amd.h $(LIB)
+	@mkdir -p $(OBJ)
+	$(CXX) $(CXXFLAGS) -Iinclude -o $@ $< -Lcufhedb_amd -lcufhedb_amd -Wl,-rpath,'$$ORIGIN/../cufhedb_amd'
 
 oracle:
 	$(MAKE) -C oracle all

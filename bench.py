@@ -118,6 +118,9 @@ def main():
     ap.add_argument("--preset", choices=["cfg636", "ref672"], default="cfg636")
     ap.add_argument("--variant", choices=["fp64"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a plain copy "
+                         "kernel of known size, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -149,6 +152,14 @@ def main():
 
     def step():
         F.check(L_.fhe_pbs_lvl01_dev(ctx.h, d_in.ptr, d_tv.ptr, 0, d_out.ptr, B, None))
+
+    if args.calib_copy_mib:
+        nb = args.calib_copy_mib << 20
+        src, dst = F.DeviceBuffer(nb), F.DeviceBuffer(nb)
+        F.check(L_.fhe_memcpy_d2d(dst.ptr, src.ptr, nb))
+        F.check(L_.fhe_device_sync())
+        src.free()
+        dst.free()
 
     for _ in range(args.warmup):
         step()

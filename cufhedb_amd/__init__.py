@@ -99,6 +99,7 @@ _SIGS = {
     "fhe_pbs_lvl01_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp]),
     "fhe_blind_rotate_lvl01_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp]),
     "fhe_pbs_lvl02_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp]),
+    "fhe_blind_rotate_lvl02_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp]),
     "fhe_sample_extract_lvl1_dev": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
     "fhe_sample_extract_lvl2_dev": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
     "fhe_iks_dev": (_i, [_vp, _i, _vp, _vp, _i, _vp]),
@@ -117,6 +118,7 @@ _SIGS = {
     "fhe_free": (_i, [_vp]),
     "fhe_memcpy_h2d": (_i, [_vp, _vp, _sz]),
     "fhe_memcpy_d2h": (_i, [_vp, _vp, _sz]),
+    "fhe_memcpy_d2d": (_i, [_vp, _vp, _sz]),
     "fhe_memset_dev": (_i, [_vp, _i, _sz]),
     "fhe_device_sync": (_i, []),
     "fhe_timer_start": (_i, [_vp, _vp]),
@@ -343,12 +345,13 @@ class Context:
         check(lib().fhe_iks(self.h, kind, _ptr(tlwe), _ptr(out), B))
         return out
 
-    def _run_dev(self, fn, inputs, out_shape, out_dtype, *extra):
-        """Copy inputs to the device, run fn(ctx, *dev_in, dev_out, *extra, stream=None),
-        copy the output back."""
+    def _run_dev(self, fn, inputs, out_shape, out_dtype, *extra, stride_first=None):
+        """Copy inputs to the device, run fn(ctx, *dev_in, [stride,] dev_out, *extra,
+        stream=None), copy the output back."""
         dins = [DeviceBuffer.from_array(a) for a in inputs]
         dout = DeviceBuffer(int(np.prod(out_shape)) * np.dtype(out_dtype).itemsize)
-        check(fn(self.h, *[d.ptr for d in dins], dout.ptr, *extra, None))
+        mid = [] if stride_first is None else [stride_first]
+        check(fn(self.h, *[d.ptr for d in dins], *mid, dout.ptr, *extra, None))
         self.sync()
         out = dout.to_array(out_dtype, out_shape)
         for d in dins:
@@ -380,6 +383,45 @@ class Context:
         return self._run_dev(lib().fhe_external_product_lvl1_dev,
                              [trlwe, np.ascontiguousarray(trgswfft, np.float64)],
                              (B, 2, LVL1_N), np.uint32, B)
+
+    def pbs_lvl02(self, tlwe0, tv):
+        tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)
+        tv = np.ascontiguousarray(tv, np.uint64)
+        B = tlwe0.shape[0]
+        stride = 0 if tv.ndim == 1 else LVL2_N
+        return self._run_dev(lib().fhe_pbs_lvl02_dev, [tlwe0, tv], (B, LVL2_N + 1), np.uint64,
+                             B, stride_first=stride)
+
+    def blind_rotate_lvl02(self, tlwe0, tv):
+        tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)
+        tv = np.ascontiguousarray(tv, np.uint64)
+        B = tlwe0.shape[0]
+        stride = 0 if tv.ndim == 1 else LVL2_N
+        return self._run_dev(lib().fhe_blind_rotate_lvl02_dev, [tlwe0, tv], (B, 2, LVL2_N),
+                             np.uint64, B, stride_first=stride)
+
+    def sample_extract_lvl2(self, trlwe, index=0):
+        trlwe = np.ascontiguousarray(trlwe, np.uint64)
+        B = trlwe.shape[0]
+        return self._run_dev(lib().fhe_sample_extract_lvl2_dev, [trlwe], (B, LVL2_N + 1),
+                             np.uint64, index, B)
+
+    def external_product_lvl2(self, trlwe, trgswfft):
+        trlwe = np.ascontiguousarray(trlwe, np.uint64)
+        B = trlwe.shape[0]
+        return self._run_dev(lib().fhe_external_product_lvl2_dev,
+                             [trlwe, np.ascontiguousarray(trgswfft, np.float64)],
+                             (B, 2, LVL2_N), np.uint64, B)
+
+    def ifft_lvl2(self, poly):
+        poly = np.ascontiguousarray(poly, np.uint64).reshape(-1, LVL2_N)
+        B = poly.shape[0]
+        return self._run_dev(lib().fhe_ifft_lvl2_dev, [poly], (B, LVL2_N), np.float64, B)
+
+    def fft_lvl2(self, fd):
+        fd = np.ascontiguousarray(fd, np.float64).reshape(-1, LVL2_N)
+        B = fd.shape[0]
+        return self._run_dev(lib().fhe_fft_lvl2_dev, [fd], (B, LVL2_N), np.uint64, B)
 
     def ifft_lvl1(self, poly):
         poly = np.ascontiguousarray(poly, np.uint32).reshape(-1, LVL1_N)

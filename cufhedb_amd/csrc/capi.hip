@@ -141,22 +141,27 @@ int fhe_load_bk(fhe_ctx *ctx, int level, const void *bk_host, int format)
     FHE_CHECK_ARG(format == FHE_BK_INT || format == FHE_BK_FFT, "fhe_load_bk: bad format %d", format);
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     const int n0 = int(ctx->p.n0);
-    if (level == 2) {
-        set_error("fhe_load_bk: level 2 is not available in this build");
-        return FHE_ERR_UNSUPPORTED;
-    }
-    const size_t elems = bk1_device_elems(n0);
-    if (!ctx->bk1) HIP_TRY(hipMalloc(&ctx->bk1, elems * sizeof(cd)), "BK allocation");
-    const size_t npoly = size_t(n0) * Lvl1::rows * 2;
-    const size_t src_bytes =
-        npoly * Lvl1::N * (format == FHE_BK_INT ? sizeof(uint32_t) : sizeof(double));
+    const bool l1 = level == 1;
+    const size_t elems = l1 ? bk1_device_elems(n0) : bk2_device_elems(n0);
+    cd *&dst = l1 ? ctx->bk1 : ctx->bk2;
+    if (!dst) HIP_TRY(hipMalloc(&dst, elems * sizeof(cd)), "BK allocation");
+    const int rows = l1 ? Lvl1::rows : Lvl2::rows;
+    const int N = l1 ? Lvl1::N : Lvl2::N;
+    const size_t word = format == FHE_BK_FFT ? sizeof(double) : (l1 ? 4 : 8);
+    const size_t src_bytes = size_t(n0) * rows * 2 * N * word;
     void *tmp = nullptr;
     HIP_TRY(hipMalloc(&tmp, src_bytes), "BK staging allocation");
     hipError_t e = hipMemcpyAsync(tmp, bk_host, src_bytes, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess)
-        e = format == FHE_BK_INT
-                ? launch_bk1_from_int(static_cast<uint32_t *>(tmp), ctx->bk1, n0, ctx->stream)
-                : launch_bk1_from_fft(static_cast<double *>(tmp), ctx->bk1, n0, ctx->stream);
+    if (e == hipSuccess) {
+        if (l1)
+            e = format == FHE_BK_INT
+                    ? launch_bk1_from_int(static_cast<uint32_t *>(tmp), dst, n0, ctx->stream)
+                    : launch_bk1_from_fft(static_cast<double *>(tmp), dst, n0, ctx->stream);
+        else
+            e = format == FHE_BK_INT
+                    ? launch_bk2_from_int(static_cast<uint64_t *>(tmp), dst, n0, ctx->stream)
+                    : launch_bk2_from_fft(static_cast<double *>(tmp), dst, n0, ctx->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(tmp);
     HIP_TRY(e, "fhe_load_bk");
@@ -207,12 +212,36 @@ int fhe_pbs_lvl01_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint32_t *tv_d
     return FHE_OK;
 }
 
-int fhe_pbs_lvl02_dev(fhe_ctx *ctx, const uint32_t *, const uint64_t *, int, uint64_t *, int,
-                      void *)
+int fhe_pbs_lvl02_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint64_t *tv_dev, int tv_stride,
+                      uint64_t *out_dev, int B, void *stream)
 {
-    FHE_CHECK_ARG(ctx, "fhe_pbs_lvl02_dev: null context");
-    set_error("fhe_pbs_lvl02_dev: level 2 is not available in this build");
-    return FHE_ERR_UNSUPPORTED;
+    FHE_CHECK_ARG(ctx && in_dev && tv_dev && out_dev, "fhe_pbs_lvl02_dev: null argument");
+    FHE_CHECK_ARG(B >= 0, "fhe_pbs_lvl02_dev: negative batch");
+    FHE_CHECK_ARG(tv_stride == 0 || tv_stride == Lvl2::N, "tv_stride must be 0 or %d", Lvl2::N);
+    if (!ctx->bk2) {
+        set_error("fhe_pbs_lvl02_dev: lvl02 bootstrapping key not loaded");
+        return FHE_ERR_NOT_LOADED;
+    }
+    HIP_TRY(launch_blind_rotate2(in_dev, int(ctx->p.n0), ctx->bk2, tv_dev, tv_stride, out_dev, true,
+                                 B, pick(ctx, stream)),
+            "pbs lvl02");
+    return FHE_OK;
+}
+
+int fhe_blind_rotate_lvl02_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint64_t *tv_dev,
+                               int tv_stride, uint64_t *acc_dev, int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && in_dev && tv_dev && acc_dev, "fhe_blind_rotate_lvl02_dev: null argument");
+    FHE_CHECK_ARG(B >= 0, "fhe_blind_rotate_lvl02_dev: negative batch");
+    FHE_CHECK_ARG(tv_stride == 0 || tv_stride == Lvl2::N, "tv_stride must be 0 or %d", Lvl2::N);
+    if (!ctx->bk2) {
+        set_error("fhe_blind_rotate_lvl02_dev: lvl02 bootstrapping key not loaded");
+        return FHE_ERR_NOT_LOADED;
+    }
+    HIP_TRY(launch_blind_rotate2(in_dev, int(ctx->p.n0), ctx->bk2, tv_dev, tv_stride, acc_dev,
+                                 false, B, pick(ctx, stream)),
+            "blind rotate lvl02");
+    return FHE_OK;
 }
 
 int fhe_sample_extract_lvl1_dev(fhe_ctx *ctx, const uint32_t *trlwe_dev, uint32_t *tlwe_dev,
@@ -225,11 +254,14 @@ int fhe_sample_extract_lvl1_dev(fhe_ctx *ctx, const uint32_t *trlwe_dev, uint32_
     return FHE_OK;
 }
 
-int fhe_sample_extract_lvl2_dev(fhe_ctx *ctx, const uint64_t *, uint64_t *, int, int, void *)
+int fhe_sample_extract_lvl2_dev(fhe_ctx *ctx, const uint64_t *trlwe_dev, uint64_t *tlwe_dev,
+                                int index, int B, void *stream)
 {
-    FHE_CHECK_ARG(ctx, "fhe_sample_extract_lvl2_dev: null context");
-    set_error("fhe_sample_extract_lvl2_dev: level 2 is not available in this build");
-    return FHE_ERR_UNSUPPORTED;
+    FHE_CHECK_ARG(ctx && trlwe_dev && tlwe_dev, "fhe_sample_extract_lvl2_dev: null argument");
+    FHE_CHECK_ARG(B >= 0 && index >= 0 && index < Lvl2::N, "fhe_sample_extract_lvl2_dev: bad size");
+    HIP_TRY(launch_sample_extract2(trlwe_dev, tlwe_dev, index, B, pick(ctx, stream)),
+            "sample extract lvl2");
+    return FHE_OK;
 }
 
 int fhe_iks_dev(fhe_ctx *ctx, int kind, const void *in_dev, uint32_t *out_dev, int B, void *stream)
@@ -265,12 +297,21 @@ int fhe_external_product_lvl1_dev(fhe_ctx *ctx, const uint32_t *trlwe_dev,
     return FHE_OK;
 }
 
-int fhe_external_product_lvl2_dev(fhe_ctx *ctx, const uint64_t *, const double *, uint64_t *, int,
-                                  void *)
+int fhe_external_product_lvl2_dev(fhe_ctx *ctx, const uint64_t *trlwe_dev,
+                                  const double *trgswfft_dev, uint64_t *out_dev, int B,
+                                  void *stream)
 {
-    FHE_CHECK_ARG(ctx, "fhe_external_product_lvl2_dev: null context");
-    set_error("fhe_external_product_lvl2_dev: level 2 is not available in this build");
-    return FHE_ERR_UNSUPPORTED;
+    FHE_CHECK_ARG(ctx && trlwe_dev && trgswfft_dev && out_dev,
+                  "fhe_external_product_lvl2_dev: null argument");
+    FHE_CHECK_ARG(B >= 0, "fhe_external_product_lvl2_dev: negative batch");
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    if (!ctx->trgsw2)
+        HIP_TRY(hipMalloc(&ctx->trgsw2, size_t(Lvl2::rows) * 2 * 1024 * sizeof(cd)),
+                "TRGSW scratch allocation");
+    hipStream_t s = pick(ctx, stream);
+    HIP_TRY(launch_trgsw2_from_fft(trgswfft_dev, ctx->trgsw2, s), "TRGSW layout");
+    HIP_TRY(launch_extprod2(trlwe_dev, ctx->trgsw2, out_dev, B, s), "external product lvl2");
+    return FHE_OK;
 }
 
 int fhe_ifft_lvl1_dev(fhe_ctx *ctx, const uint32_t *poly_dev, double *fd_dev, int B, void *stream)
@@ -287,18 +328,18 @@ int fhe_fft_lvl1_dev(fhe_ctx *ctx, const double *fd_dev, uint32_t *poly_dev, int
     return FHE_OK;
 }
 
-int fhe_ifft_lvl2_dev(fhe_ctx *ctx, const uint64_t *, double *, int, void *)
+int fhe_ifft_lvl2_dev(fhe_ctx *ctx, const uint64_t *poly_dev, double *fd_dev, int B, void *stream)
 {
-    FHE_CHECK_ARG(ctx, "fhe_ifft_lvl2_dev: null context");
-    set_error("fhe_ifft_lvl2_dev: level 2 is not available in this build");
-    return FHE_ERR_UNSUPPORTED;
+    FHE_CHECK_ARG(ctx && poly_dev && fd_dev && B >= 0, "fhe_ifft_lvl2_dev: bad argument");
+    HIP_TRY(launch_ifft2(poly_dev, fd_dev, B, pick(ctx, stream)), "ifft lvl2");
+    return FHE_OK;
 }
 
-int fhe_fft_lvl2_dev(fhe_ctx *ctx, const double *, uint64_t *, int, void *)
+int fhe_fft_lvl2_dev(fhe_ctx *ctx, const double *fd_dev, uint64_t *poly_dev, int B, void *stream)
 {
-    FHE_CHECK_ARG(ctx, "fhe_fft_lvl2_dev: null context");
-    set_error("fhe_fft_lvl2_dev: level 2 is not available in this build");
-    return FHE_ERR_UNSUPPORTED;
+    FHE_CHECK_ARG(ctx && poly_dev && fd_dev && B >= 0, "fhe_fft_lvl2_dev: bad argument");
+    HIP_TRY(launch_fft2(fd_dev, poly_dev, B, pick(ctx, stream)), "fft lvl2");
+    return FHE_OK;
 }
 
 // ---- bootstrap operators --------------------------------------------------
@@ -464,6 +505,13 @@ int fhe_memcpy_d2h(void *dst, const void *src, size_t bytes)
 {
     FHE_CHECK_ARG(dst && src, "fhe_memcpy_d2h: null pointer");
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    return FHE_OK;
+}
+
+int fhe_memcpy_d2d(void *dst, const void *src, size_t bytes)
+{
+    FHE_CHECK_ARG(dst && src, "fhe_memcpy_d2d: null pointer");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
     return FHE_OK;
 }
 

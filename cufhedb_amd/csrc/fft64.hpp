@@ -137,22 +137,38 @@ __device__ __forceinline__ Twiddles1024 make_twiddles1024(int lane)
     return t;
 }
 
+// bases for the plain (untwisted) 512-point DFT
+__device__ __forceinline__ Twiddles1024 make_twiddles512_plain(int lane)
+{
+    Twiddles1024 t;
+    t.wl = cd{1.0, 0.0};
+    t.w4l = unit_root(double(lane), 256.0);
+    t.w64 = unit_root(double(lane & 7), 32.0);
+    return t;
+}
+
 constexpr int kFftPad = 65;                 // LDS row stride in double2
 constexpr int kFftLdsElems = 8 * kFftPad;   // 520 double2 = 8320 bytes
 
 // Forward transform: v[j] = z[L + 64 j] on entry, Z in permuted layout on exit.
+// TWIST = true: twisted negacyclic map above (w = exp(i pi/1024)).
+// TWIST = false: plain DFT of size 512, Z[m] = sum_n z_n exp(2 pi i n m/512)
+//                (tw.wl must then be 1).
+template <bool TWIST = true>
 __device__ __forceinline__ void fft512_fwd(cd v[8], cd *lds, int lane, const Twiddles1024 &tw)
 {
     // twist constants, pass 1
+    if (TWIST) {
 #pragma unroll
-    for (int j = 1; j < 8; j++) v[j] = cmul(v[j], twist_const(j));
+        for (int j = 1; j < 8; j++) v[j] = cmul(v[j], twist_const(j));
+    }
     dft8<1>(v);
     {
         cd t = tw.wl;
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            v[k] = cmul(v[k], t);
-            if (k < 7) t = cmul(t, tw.w4l);
+            if (TWIST || k > 0) v[k] = cmul(v[k], t);
+            if (k < 7) t = (TWIST || k > 0) ? cmul(t, tw.w4l) : tw.w4l;
         }
     }
     // transpose 1: T[k1][L] -> lane (L0 + 8 k1) register L1
@@ -189,6 +205,7 @@ __device__ __forceinline__ void fft512_fwd(cd v[8], cd *lds, int lane, const Twi
 }
 
 // Inverse transform (unnormalised: returns 512 * z): permuted Z in, natural z out.
+template <bool TWIST = true>
 __device__ __forceinline__ void fft512_inv(cd v[8], cd *lds, int lane, const Twiddles1024 &tw)
 {
     dft8<-1>(v);
@@ -224,13 +241,15 @@ __device__ __forceinline__ void fft512_inv(cd v[8], cd *lds, int lane, const Twi
         cd t = tw.wl;
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            v[k] = cmulc(v[k], t);
-            if (k < 7) t = cmul(t, tw.w4l);
+            if (TWIST || k > 0) v[k] = cmulc(v[k], t);
+            if (k < 7) t = (TWIST || k > 0) ? cmul(t, tw.w4l) : tw.w4l;
         }
     }
     dft8<-1>(v);
+    if (TWIST) {
 #pragma unroll
-    for (int j = 1; j < 8; j++) v[j] = cmulc(v[j], twist_const(j));
+        for (int j = 1; j < 8; j++) v[j] = cmulc(v[j], twist_const(j));
+    }
 }
 
 // frequency index held by (lane s, register q) in the permuted layout

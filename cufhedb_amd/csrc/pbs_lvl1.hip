@@ -91,26 +91,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 #pragma unroll
         for (int q = 0; q < 8; q++) fd0[q] = fd1[q] = cd{0.0, 0.0};
 #pragma unroll 1
-        for (int c = 0; c < 2; c++) {
+        for (int r = 0; r < Lvl1::rows; r++) {
+            const int c = r / Lvl1::l, d = r - c * Lvl1::l;
             const uint32_t *poly = acc + c * N1;
-            uint32_t t[16];
+            // issue this row's key loads before the FFT so their latency
+            // overlaps the transform
+            cd b0[8], b1[8];
+            load_row(bki + r * 2 * kPolyCd, lane, b0, b1);
+            // digits of (X^abar - 1) acc_c, recomputed per digit from LDS
+            cd v[8];
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                t[2 * j] = rot_minus_one(poly, abar, lane + 64 * j);
-                t[2 * j + 1] = rot_minus_one(poly, abar, lane + 64 * j + 512);
-            }
-#pragma unroll 1
-            for (int d = 0; d < Lvl1::l; d++) {
-                const cd *row = bki + (c * Lvl1::l + d) * 2 * kPolyCd;
-                cd v[8];
+            for (int j = 0; j < 8; j++)
+                v[j] = cd{digit1(rot_minus_one(poly, abar, lane + 64 * j), d),
+                          digit1(rot_minus_one(poly, abar, lane + 64 * j + 512), d)};
+            fft512_fwd(v, tile, lane, tw);
 #pragma unroll
-                for (int j = 0; j < 8; j++) v[j] = cd{digit1(t[2 * j], d), digit1(t[2 * j + 1], d)};
-                fft512_fwd(v, tile, lane, tw);
-#pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    cmac(fd0[q], v[q], row[q * 64 + lane]);
-                    cmac(fd1[q], v[q], row[8 * 64 + q * 64 + lane]);
-                }
+            for (int q = 0; q < 8; q++) {
+                cmac(fd0[q], v[q], b0[q]);
+                cmac(fd1[q], v[q], b1[q]);
             }
         }
         // every lane's rotation reads are complete (the FFTs synchronised)

@@ -146,6 +146,9 @@ int fhe_blind_rotate_lvl01_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint3
  * BlindRotate_gpu.cuh:534-670), uint64_t test vectors / outputs. */
 int fhe_pbs_lvl02_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint64_t *tv_dev, int tv_stride,
                       uint64_t *out_dev, int B, void *stream);
+/* Blind rotation only, lvl2 accumulator (B x 2 x 2048 uint64_t). */
+int fhe_blind_rotate_lvl02_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint64_t *tv_dev,
+                               int tv_stride, uint64_t *acc_dev, int B, void *stream);
 /* SampleExtractIndex (thirdparty/TFHEpp/src/trlwe.cpp:109-118;
  * BlindRotate_gpu.cuh:386-397,703-718): B TRLWEs -> B TLWEs. */
 int fhe_sample_extract_lvl1_dev(fhe_ctx *ctx, const uint32_t *trlwe_dev, uint32_t *tlwe_dev,
@@ -207,6 +210,7 @@ int fhe_malloc(void **ptr, size_t bytes);
 int fhe_free(void *ptr);
 int fhe_memcpy_h2d(void *dst_dev, const void *src, size_t bytes);
 int fhe_memcpy_d2h(void *dst, const void *src_dev, size_t bytes);
+int fhe_memcpy_d2d(void *dst_dev, const void *src_dev, size_t bytes);
 int fhe_memset_dev(void *dst_dev, int value, size_t bytes);
 int fhe_device_sync(void);
 /* Elapsed milliseconds of the work enqueued on `stream` between

@@ -1,0 +1,341 @@
+// cuhedb.hpp -- header-only C++ drop-in for cuFHEDB's GPU bootstrapping
+// surface, implemented on the C ABI of libcufhedb_amd.so (cufhedb_amd.h).
+//
+// Mirrors, with the same names, argument order and meaning:
+//   cufft::CuFFT_Torus<P>           src/cuHEDB/fft_gpu/cufft_gpu.cuh:10-451
+//     LoadBK, GateBootstrappingTLWE2TLWEFFT_st, IdeGateBootstrappingTLWE2TLWEFFT_st,
+//     ExternalProduct_st, ifft_st, fft_st
+//   cuHEDB:: operators               src/cuHEDB/gatebootstrap_gpu.cuh:31-170
+//     MSBGateBootstrapping, IdeGateBootstrapping, HomNOT, HomAND, HomOR,
+//     LOG_to_ARI, log_rescale
+// P is any parameter struct with TFHEpp's member names (n, k, l, Bgbit, T):
+// TFHEpp::lvl1param / lvl2param work unchanged, and cufhedb_amd::lvl1param /
+// lvl2param below are provided for code that does not include TFHEpp.
+//
+// Differences a caller must know:
+//   * Keys are loaded once into a Runtime (device-resident); the operators
+//     take no EvalKey argument.  The reference's operators take the EvalKey
+//     and read *ek.iksklvl10 on the host; here the key switch runs on the GPU
+//     with the key uploaded by Runtime::load_ksk.
+//   * stream_id is accepted for source compatibility; every call is
+//     synchronous, as in the reference, and calls serialise on the context.
+//   * Batched forms (the *_batch members) take B ciphertexts per call; that
+//     is the fast path.
+//   * Errors throw cufhedb_amd::Error (the reference ignores CUDA errors).
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../cufhedb_amd.h"
+
+namespace cufhedb_amd {
+
+struct lvl1param {
+    static constexpr uint32_t nbit = 10, n = 1u << nbit, k = 1, l = 3, Bgbit = 6;
+    using T = uint32_t;
+    static constexpr T mu = 1u << 29;
+};
+struct lvl2param {
+    static constexpr uint32_t nbit = 11, n = 1u << nbit, k = 1, l = 4, Bgbit = 9;
+    using T = uint64_t;
+    static constexpr T mu = 1ull << 61;
+};
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc, const char *what)
+{
+    if (rc != FHE_OK) throw Error(rc, std::string(what) + ": " + fhe_last_error());
+}
+
+// RAII device buffer
+class DeviceBuffer {
+  public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t bytes) { reset(bytes); }
+    ~DeviceBuffer() { fhe_free(p_); }
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    void reset(size_t bytes)
+    {
+        if (bytes <= cap_) return;
+        fhe_free(p_);
+        p_ = nullptr;
+        check(fhe_malloc(&p_, bytes), "fhe_malloc");
+        cap_ = bytes;
+    }
+    void *get() const { return p_; }
+    template <class U>
+    U *as() const { return static_cast<U *>(p_); }
+    void upload(const void *src, size_t bytes) { check(fhe_memcpy_h2d(p_, src, bytes), "H2D"); }
+    void download(void *dst, size_t bytes) const { check(fhe_memcpy_d2h(dst, p_, bytes), "D2H"); }
+
+  private:
+    void *p_ = nullptr;
+    size_t cap_ = 0;
+};
+
+// One GPU context with resident keys (the reference's global singletons
+// cufftlvl1 / cufftlvl2, gatebootstrap_gpu.cuh:24-25).
+class Runtime {
+  public:
+    explicit Runtime(const fhe_params &p, int device = 0) : params_(p)
+    {
+        check(fhe_ctx_create(&params_, device, &ctx_), "fhe_ctx_create");
+    }
+    explicit Runtime(int preset = FHE_PRESET_REF672, int device = 0)
+    {
+        check(fhe_params_preset(preset, &params_), "fhe_params_preset");
+        check(fhe_ctx_create(&params_, device, &ctx_), "fhe_ctx_create");
+    }
+    ~Runtime() { fhe_ctx_destroy(ctx_); }
+    Runtime(const Runtime &) = delete;
+    Runtime &operator=(const Runtime &) = delete;
+
+    fhe_ctx *ctx() const { return ctx_; }
+    const fhe_params &params() const { return params_; }
+    uint32_t n0() const { return params_.n0; }
+
+    void load_bk(int level, const void *bk, int format) { check(fhe_load_bk(ctx_, level, bk, format), "fhe_load_bk"); }
+    void load_ksk(int kind, const uint32_t *ksk) { check(fhe_load_ksk(ctx_, kind, ksk), "fhe_load_ksk"); }
+
+  private:
+    fhe_params params_{};
+    fhe_ctx *ctx_ = nullptr;
+};
+
+}  // namespace cufhedb_amd
+
+namespace cufft {
+
+// Same member names as the reference class; P::n selects lvl1 or lvl2.
+template <class P>
+class CuFFT_Torus {
+    static_assert(P::n == 1024 || P::n == 2048, "lvl1 (N=1024) or lvl2 (N=2048)");
+    using T = typename P::T;
+    static constexpr int kLevel = P::n == 1024 ? 1 : 2;
+
+  public:
+    const int32_t _2N = 2 * P::n, N = P::n, Ns2 = P::n / 2;
+
+    // num_stream kept for source compatibility (one context serves all ids)
+    explicit CuFFT_Torus(cufhedb_amd::Runtime &rt, const int num_stream = 20)
+        : rt_(rt), num_stream_(num_stream) {}
+
+    // LoadBK (cufft_gpu.cuh:175-184): TFHEpp BootstrappingKeyFFT, any n0
+    template <size_t NK>
+    void LoadBK(const std::array<std::array<std::array<std::array<double, P::n>, 2>,
+                                            (P::k + 1) * P::l>, NK> &bkfft)
+    {
+        if (NK != rt_.n0()) throw cufhedb_amd::Error(FHE_ERR_ARG, "LoadBK: key dimension != n0");
+        rt_.load_bk(kLevel, bkfft.data(), FHE_BK_FFT);
+    }
+    void LoadBK(const double *bkfft) { rt_.load_bk(kLevel, bkfft, FHE_BK_FFT); }
+    void LoadBKInt(const T *bk) { rt_.load_bk(kLevel, bk, FHE_BK_INT); }
+
+    // GateBootstrappingTLWE2TLWEFFT_st (cufft_gpu.cuh:250-271): constant test
+    // vector u, lvl0 input of n0+1 words.
+    template <class In>
+    void GateBootstrappingTLWE2TLWEFFT_st(std::array<T, P::n + 1> &tlwe, const In &in, const T u,
+                                          const uint32_t /*stream_id*/)
+    {
+        std::vector<T> tv(P::n, u);
+        pbs_batch(tlwe.data(), in.data(), tv.data(), 0, 1);
+    }
+    // IdeGateBootstrappingTLWE2TLWEFFT_st (cufft_gpu.cuh:273-294): test
+    // vector 2^scale_bits * (i >> 6)  (BlindRotate_gpu.cuh:124-161)
+    template <class In>
+    void IdeGateBootstrappingTLWE2TLWEFFT_st(std::array<T, P::n + 1> &tlwe, const In &in,
+                                             const uint32_t scale_bits, const uint32_t /*stream_id*/)
+    {
+        std::vector<T> tv(P::n);
+        for (uint32_t i = 0; i < P::n; i++) tv[i] = (T(1) << scale_bits) * T(i >> 6);
+        pbs_batch(tlwe.data(), in.data(), tv.data(), 0, 1);
+    }
+    // ExternalProduct_st (cufft_gpu.cuh:228-248)
+    void ExternalProduct_st(std::array<std::array<T, P::n>, 2> &res,
+                            const std::array<std::array<T, P::n>, 2> &trlwe,
+                            const std::array<std::array<std::array<double, P::n>, 2>,
+                                             (P::k + 1) * P::l> &trgswfft,
+                            const uint32_t /*stream_id*/)
+    {
+        external_product_batch(res[0].data(), trlwe[0].data(), &trgswfft[0][0][0], 1);
+    }
+    // ifft_st / fft_st (cufft_gpu.cuh:187-212): TwistIFFT / TwistFFT
+    void ifft_st(std::array<double, P::n> &res, const std::array<T, P::n> &a, const uint32_t)
+    {
+        transform(res.data(), a.data(), 1, true);
+    }
+    void fft_st(std::array<T, P::n> &res, const std::array<double, P::n> &a, const uint32_t)
+    {
+        transform(res.data(), a.data(), 1, false);
+    }
+
+    // ---- batched forms (host arrays) -------------------------------------
+    // B lvl0 TLWEs (n0+1 words each) -> B lvl P TLWEs; tv: N words
+    // (tv_stride 0) or B x N (tv_stride N)
+    void pbs_batch(T *out, const uint32_t *in, const T *tv, int tv_stride, int B)
+    {
+        const size_t in_b = size_t(B) * (rt_.n0() + 1) * 4, out_b = size_t(B) * (P::n + 1) * sizeof(T);
+        const size_t tv_b = (tv_stride ? size_t(B) : 1) * P::n * sizeof(T);
+        din_.reset(in_b);
+        dtv_.reset(tv_b);
+        dout_.reset(out_b);
+        din_.upload(in, in_b);
+        dtv_.upload(tv, tv_b);
+        if constexpr (kLevel == 1)
+            cufhedb_amd::check(fhe_pbs_lvl01_dev(rt_.ctx(), din_.as<uint32_t>(), dtv_.as<uint32_t>(),
+                                                 tv_stride, dout_.as<uint32_t>(), B, nullptr),
+                               "fhe_pbs_lvl01_dev");
+        else
+            cufhedb_amd::check(fhe_pbs_lvl02_dev(rt_.ctx(), din_.as<uint32_t>(), dtv_.as<uint64_t>(),
+                                                 tv_stride, dout_.as<uint64_t>(), B, nullptr),
+                               "fhe_pbs_lvl02_dev");
+        cufhedb_amd::check(fhe_ctx_sync(rt_.ctx()), "fhe_ctx_sync");
+        dout_.download(out, out_b);
+    }
+    void external_product_batch(T *out, const T *trlwe, const double *trgswfft, int B)
+    {
+        const size_t io_b = size_t(B) * 2 * P::n * sizeof(T);
+        const size_t g_b = size_t(2 * P::l) * 2 * P::n * sizeof(double);
+        din_.reset(io_b);
+        dtv_.reset(g_b);
+        dout_.reset(io_b);
+        din_.upload(trlwe, io_b);
+        dtv_.upload(trgswfft, g_b);
+        if constexpr (kLevel == 1)
+            cufhedb_amd::check(fhe_external_product_lvl1_dev(rt_.ctx(), din_.as<uint32_t>(),
+                                                             dtv_.as<double>(), dout_.as<uint32_t>(),
+                                                             B, nullptr),
+                               "fhe_external_product_lvl1_dev");
+        else
+            cufhedb_amd::check(fhe_external_product_lvl2_dev(rt_.ctx(), din_.as<uint64_t>(),
+                                                             dtv_.as<double>(), dout_.as<uint64_t>(),
+                                                             B, nullptr),
+                               "fhe_external_product_lvl2_dev");
+        cufhedb_amd::check(fhe_ctx_sync(rt_.ctx()), "fhe_ctx_sync");
+        dout_.download(out, io_b);
+    }
+
+  private:
+    template <class A, class Bp>
+    void transform(A *dst, const Bp *src, int B, bool to_fd)
+    {
+        const size_t tb = size_t(B) * P::n * sizeof(T), fb = size_t(B) * P::n * sizeof(double);
+        din_.reset(to_fd ? tb : fb);
+        dout_.reset(to_fd ? fb : tb);
+        din_.upload(src, to_fd ? tb : fb);
+        int rc;
+        if (to_fd)
+            rc = kLevel == 1 ? fhe_ifft_lvl1_dev(rt_.ctx(), din_.as<uint32_t>(), dout_.as<double>(), B, nullptr)
+                             : fhe_ifft_lvl2_dev(rt_.ctx(), din_.as<uint64_t>(), dout_.as<double>(), B, nullptr);
+        else
+            rc = kLevel == 1 ? fhe_fft_lvl1_dev(rt_.ctx(), din_.as<double>(), dout_.as<uint32_t>(), B, nullptr)
+                             : fhe_fft_lvl2_dev(rt_.ctx(), din_.as<double>(), dout_.as<uint64_t>(), B, nullptr);
+        cufhedb_amd::check(rc, "transform");
+        cufhedb_amd::check(fhe_ctx_sync(rt_.ctx()), "fhe_ctx_sync");
+        dout_.download(dst, to_fd ? fb : tb);
+    }
+
+    cufhedb_amd::Runtime &rt_;
+    int num_stream_;
+    cufhedb_amd::DeviceBuffer din_, dtv_, dout_;
+};
+
+}  // namespace cufft
+
+namespace cuHEDB {
+
+// result_type: true = LOGIC (+-1/8), false = ARITHMETIC (0 / 1/2),
+// HEDB/utils/types.h:4-11
+constexpr bool LOGIC = true;
+constexpr bool ARITHMETIC = false;
+
+using TLWELvl1 = std::array<uint32_t, cufhedb_amd::lvl1param::n + 1>;
+
+namespace detail {
+inline void run_unary(cufhedb_amd::Runtime &rt,
+                      int (*fn)(fhe_ctx *, const uint32_t *, uint32_t *, int, int, void *),
+                      uint32_t *out, const uint32_t *in, int arg, int B)
+{
+    const size_t bytes = size_t(B) * (cufhedb_amd::lvl1param::n + 1) * 4;
+    cufhedb_amd::DeviceBuffer din(bytes), dout(bytes);
+    din.upload(in, bytes);
+    cufhedb_amd::check(fn(rt.ctx(), din.as<uint32_t>(), dout.as<uint32_t>(), arg, B, nullptr), "operator");
+    cufhedb_amd::check(fhe_ctx_sync(rt.ctx()), "fhe_ctx_sync");
+    dout.download(out, bytes);
+}
+inline int ide_adapter(fhe_ctx *c, const uint32_t *i, uint32_t *o, int scale, int B, void *s)
+{
+    return fhe_ide_gate_lvl1_dev(c, i, o, uint32_t(scale), B, s);
+}
+inline void run_gate(cufhedb_amd::Runtime &rt, int gate, uint32_t *out, const uint32_t *a,
+                     const uint32_t *b, bool result_type, int B)
+{
+    const size_t bytes = size_t(B) * (cufhedb_amd::lvl1param::n + 1) * 4;
+    cufhedb_amd::DeviceBuffer da(bytes), db(bytes), dout(bytes);
+    da.upload(a, bytes);
+    db.upload(b, bytes);
+    cufhedb_amd::check(fhe_gate_lvl1_dev(rt.ctx(), gate, da.as<uint32_t>(), db.as<uint32_t>(),
+                                         dout.as<uint32_t>(), result_type ? 1 : 0, B, nullptr),
+                       "fhe_gate_lvl1_dev");
+    cufhedb_amd::check(fhe_ctx_sync(rt.ctx()), "fhe_ctx_sync");
+    dout.download(out, bytes);
+}
+}  // namespace detail
+
+// MSBGateBootstrapping (gatebootstrap_gpu.cuh:31-46), lvl1 -> lvl1
+inline void MSBGateBootstrapping(cufhedb_amd::Runtime &rt, TLWELvl1 &res, const TLWELvl1 &tlwe,
+                                 bool result_type, uint32_t /*stream_id*/ = 0)
+{
+    detail::run_unary(rt, fhe_msb_gate_lvl1_dev, res.data(), tlwe.data(), result_type ? 1 : 0, 1);
+}
+// IdeGateBootstrapping (gatebootstrap_gpu.cuh:82-92), lvl1 -> lvl1
+inline void IdeGateBootstrapping(cufhedb_amd::Runtime &rt, TLWELvl1 &res, const TLWELvl1 &tlwe,
+                                 uint32_t scale_bits, uint32_t /*stream_id*/ = 0)
+{
+    detail::run_unary(rt, detail::ide_adapter, res.data(), tlwe.data(), int(scale_bits), 1);
+}
+// HomNOT (gatebootstrap_gpu.cuh:117-121): no bootstrap
+template <class P>
+inline void HomNOT(std::array<typename P::T, P::n + 1> &res, const std::array<typename P::T, P::n + 1> &tlwe)
+{
+    for (uint32_t i = 0; i <= P::n; i++) res[i] = typename P::T(0) - tlwe[i];
+}
+// HomAND / HomOR (gatebootstrap_gpu.cuh:123-153)
+inline void HomAND(cufhedb_amd::Runtime &rt, TLWELvl1 &res, const TLWELvl1 &ca, const TLWELvl1 &cb,
+                   bool result_type, uint32_t /*stream_id*/ = 0)
+{
+    detail::run_gate(rt, FHE_GATE_AND, res.data(), ca.data(), cb.data(), result_type, 1);
+}
+inline void HomOR(cufhedb_amd::Runtime &rt, TLWELvl1 &res, const TLWELvl1 &ca, const TLWELvl1 &cb,
+                  bool result_type, uint32_t /*stream_id*/ = 0)
+{
+    detail::run_gate(rt, FHE_GATE_OR, res.data(), ca.data(), cb.data(), result_type, 1);
+}
+// batched forms: B records back to back
+inline void HomAND_batch(cufhedb_amd::Runtime &rt, uint32_t *res, const uint32_t *ca,
+                         const uint32_t *cb, bool result_type, int B)
+{
+    detail::run_gate(rt, FHE_GATE_AND, res, ca, cb, result_type, B);
+}
+inline void HomOR_batch(cufhedb_amd::Runtime &rt, uint32_t *res, const uint32_t *ca,
+                        const uint32_t *cb, bool result_type, int B)
+{
+    detail::run_gate(rt, FHE_GATE_OR, res, ca, cb, result_type, B);
+}
+inline void MSBGateBootstrapping_batch(cufhedb_amd::Runtime &rt, uint32_t *res, const uint32_t *in,
+                                       bool result_type, int B)
+{
+    detail::run_unary(rt, fhe_msb_gate_lvl1_dev, res, in, result_type ? 1 : 0, B);
+}
+
+}  // namespace cuHEDB

@@ -140,9 +140,9 @@ def bk_to_fft32(bk):
 
 def external_product32(trlwe, trgswfft, l=3, Bgbit=6):
     trlwe = np.ascontiguousarray(trlwe, np.uint32)
+    fd = np.ascontiguousarray(trgswfft, np.float64)
     out = np.zeros_like(trlwe)
-    orc().orc_external_product_u32(P(out), P(trlwe), P(np.ascontiguousarray(trgswfft)),
-                                   trlwe.shape[-1], l, Bgbit)
+    orc().orc_external_product_u32(P(out), P(trlwe), P(fd), trlwe.shape[-1], l, Bgbit)
     return out
 
 
@@ -150,8 +150,9 @@ def blind_rotate32(tlwe0, bkfft, tv, nbit=10, l=3, Bgbit=6):
     tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)
     N = tv.shape[0]
     out = np.zeros((2, N), np.uint32)
-    orc().orc_blind_rotate_u32(P(out), P(tlwe0), tlwe0.shape[0] - 1, P(bkfft),
-                               P(np.ascontiguousarray(tv, np.uint32)), N, nbit, l, Bgbit)
+    tv = np.ascontiguousarray(tv, np.uint32)
+    orc().orc_blind_rotate_u32(P(out), P(tlwe0), tlwe0.shape[0] - 1, P(bkfft), P(tv), N, nbit,
+                               l, Bgbit)
     return out
 
 
@@ -159,8 +160,8 @@ def pbs32(tlwe0, bkfft, tv, nbit=10, l=3, Bgbit=6):
     tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)
     N = tv.shape[0]
     out = np.zeros(N + 1, np.uint32)
-    orc().orc_pbs_u32(P(out), P(tlwe0), tlwe0.shape[0] - 1, P(bkfft),
-                      P(np.ascontiguousarray(tv, np.uint32)), N, nbit, l, Bgbit)
+    tv = np.ascontiguousarray(tv, np.uint32)
+    orc().orc_pbs_u32(P(out), P(tlwe0), tlwe0.shape[0] - 1, P(bkfft), P(tv), N, nbit, l, Bgbit)
     return out
 
 
@@ -188,15 +189,17 @@ def iks64(tlwe, ksk, tn, t, basebit):
 
 def trlwe_phase32(c, key):
     c = np.ascontiguousarray(c, np.uint32).reshape(2, -1)
+    key = np.ascontiguousarray(key, np.uint32)
     out = np.zeros(c.shape[1], np.uint32)
-    orc().orc_trlwe_phase_u32(P(out), P(c), P(np.ascontiguousarray(key, np.uint32)), c.shape[1])
+    orc().orc_trlwe_phase_u32(P(out), P(c), P(key), c.shape[1])
     return out
 
 
 def hom_nand(ca, cb, ksk10, n0, bkfft):
     out = np.zeros(1025, np.uint32)
-    orc().orc_hom_nand_lvl1(P(out), P(np.ascontiguousarray(ca, np.uint32)),
-                            P(np.ascontiguousarray(cb, np.uint32)), P(ksk10), n0, P(bkfft))
+    ca = np.ascontiguousarray(ca, np.uint32)
+    cb = np.ascontiguousarray(cb, np.uint32)
+    orc().orc_hom_nand_lvl1(P(out), P(ca), P(cb), P(ksk10), n0, P(bkfft))
     return out
 
 
@@ -208,3 +211,68 @@ def tdiff32(a, b):
 
 def tdiff64(a, b):
     return (np.asarray(a, np.uint64) - np.asarray(b, np.uint64)).view(np.int64)
+
+
+# ---------------------------------------------------------------- lvl2 (64-bit)
+def trgsw_to_fft64(trgsw, l=4):
+    trgsw = np.ascontiguousarray(trgsw, np.uint64)
+    out = np.zeros(trgsw.shape, np.float64)
+    orc().orc_trgsw_to_fft_u64(P(out), P(trgsw), trgsw.shape[-1], l)
+    return out
+
+
+def bk_to_fft64(bk):
+    out = np.zeros(bk.shape, np.float64)
+    for i in range(bk.shape[0]):
+        out[i] = trgsw_to_fft64(bk[i])
+    return out
+
+
+def external_product64(trlwe, trgswfft, l=4, Bgbit=9):
+    trlwe = np.ascontiguousarray(trlwe, np.uint64)
+    fd = np.ascontiguousarray(trgswfft, np.float64)
+    out = np.zeros_like(trlwe)
+    orc().orc_external_product_u64(P(out), P(trlwe), P(fd), trlwe.shape[-1], l, Bgbit)
+    return out
+
+
+def pbs64(tlwe0, bkfft, tv, nbit=11, l=4, Bgbit=9):
+    tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)
+    tv = np.ascontiguousarray(tv, np.uint64)
+    N = tv.shape[0]
+    out = np.zeros(N + 1, np.uint64)
+    orc().orc_pbs_u64(P(out), P(tlwe0), tlwe0.shape[0] - 1, P(bkfft), P(tv), N, nbit, l, Bgbit)
+    return out
+
+
+def blind_rotate64(tlwe0, bkfft, tv, nbit=11, l=4, Bgbit=9):
+    tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)
+    tv = np.ascontiguousarray(tv, np.uint64)
+    N = tv.shape[0]
+    out = np.zeros((2, N), np.uint64)
+    orc().orc_blind_rotate_u64(P(out), P(tlwe0), tlwe0.shape[0] - 1, P(bkfft), P(tv), N, nbit,
+                               l, Bgbit)
+    return out
+
+
+def sample_extract64(trlwe, index=0):
+    trlwe = np.ascontiguousarray(trlwe, np.uint64)
+    N = trlwe.shape[-1]
+    out = np.zeros(N + 1, np.uint64)
+    orc().orc_sample_extract_u64(P(out), P(trlwe), index, N)
+    return out
+
+
+def trlwe_phase64(c, key):
+    c = np.ascontiguousarray(c, np.uint64).reshape(2, -1)
+    key = np.ascontiguousarray(key, np.uint64)
+    out = np.zeros(c.shape[1], np.uint64)
+    orc().orc_trlwe_phase_u64(P(out), P(c), P(key), c.shape[1])
+    return out
+
+
+def tlwe_phase64(c, key):
+    c = np.atleast_2d(np.asarray(c, np.uint64))
+    n = key.shape[0]
+    dot = (c[:, :n] * np.asarray(key, np.uint64)[None, :]).sum(axis=1, dtype=np.uint64)
+    return c[:, n] - dot

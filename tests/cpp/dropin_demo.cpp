@@ -1,0 +1,75 @@
+// Exercises the C++ drop-in (include/cufhedb_amd/cuhedb.hpp) the way a
+// cuFHEDB caller would: CuFFT_Torus<P>::LoadBK / *_st calls and the cuHEDB::
+// operators.  Exit status 0 = every check passed.  Needs a GPU.
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "cufhedb_amd/cuhedb.hpp"
+
+using cufhedb_amd::lvl1param;
+
+static uint32_t phase(const uint32_t *c, const uint32_t *key, uint32_t n)
+{
+    uint32_t p = c[n];
+    for (uint32_t i = 0; i < n; i++) p -= c[i] * key[i];
+    return p;
+}
+
+int main()
+{
+    fhe_params p;
+    cufhedb_amd::check(fhe_params_preset(FHE_PRESET_CFG636, &p), "preset");
+    const uint32_t n0 = p.n0;
+    std::vector<uint32_t> s0(n0), s1(1024);
+    std::vector<uint64_t> s2(2048);
+    cufhedb_amd::check(fhe_keygen_secret(&p, 7, s0.data(), s1.data(), s2.data()), "keygen");
+    std::vector<uint32_t> bk(size_t(n0) * 6 * 2 * 1024);
+    cufhedb_amd::check(fhe_keygen_bk_lvl01(&p, 8, s0.data(), s1.data(), bk.data()), "bk");
+    std::vector<uint32_t> ksk(fhe_ksk_words(&p, 10));
+    cufhedb_amd::check(fhe_keygen_ksk(&p, 10, 9, s1.data(), s0.data(), ksk.data()), "ksk");
+
+    cufhedb_amd::Runtime rt(p, 0);
+    rt.load_ksk(10, ksk.data());
+    cufft::CuFFT_Torus<lvl1param> cufftlvl1(rt);
+    cufftlvl1.LoadBKInt(bk.data());
+
+    int bad = 0;
+    std::mt19937 gen(3);
+    // GateBootstrappingTLWE2TLWEFFT_st with constant test vector u: output
+    // +u for phase in [0, 1/2), -u otherwise
+    for (int t = 0; t < 8; t++) {
+        const int bit = gen() & 1;
+        std::vector<uint32_t> c0(n0 + 1);
+        const uint32_t m = bit ? (1u << 30) : (3u << 30);
+        cufhedb_amd::check(fhe_tlwe_encrypt_u32(11, t, s0.data(), n0, &m, p.alpha0, c0.data(), 1), "enc");
+        std::array<uint32_t, 1025> out;
+        cufftlvl1.GateBootstrappingTLWE2TLWEFFT_st(out, c0, lvl1param::mu, 0);
+        const bool pos = int32_t(phase(out.data(), s1.data(), 1024)) > 0;
+        bad += pos != bool(bit);
+    }
+    // ifft_st / fft_st round trip within one unit (TFHEpp/test/polymul.cpp:19-28)
+    std::array<uint32_t, 1024> a, back;
+    std::array<double, 1024> fd;
+    for (auto &x : a) x = gen();
+    cufftlvl1.ifft_st(fd, a, 0);
+    cufftlvl1.fft_st(back, fd, 0);
+    for (int i = 0; i < 1024; i++) bad += (int32_t(back[i] - a[i]) > 1 || int32_t(a[i] - back[i]) > 1);
+    // cuHEDB::HomAND / HomOR (LOGIC) truth tables
+    for (int t = 0; t < 8; t++) {
+        const int x = t & 1, y = (t >> 1) & 1;
+        cuHEDB::TLWELvl1 ca, cb, r;
+        const uint32_t mx = x ? lvl1param::mu : 0u - lvl1param::mu;
+        const uint32_t my = y ? lvl1param::mu : 0u - lvl1param::mu;
+        cufhedb_amd::check(fhe_tlwe_encrypt_u32(12, t, s1.data(), 1024, &mx, p.alpha1, ca.data(), 1), "enc");
+        cufhedb_amd::check(fhe_tlwe_encrypt_u32(13, t, s1.data(), 1024, &my, p.alpha1, cb.data(), 1), "enc");
+        cuHEDB::HomAND(rt, r, ca, cb, cuHEDB::LOGIC, 0);
+        bad += (int32_t(phase(r.data(), s1.data(), 1024)) > 0) != bool(x & y);
+        cuHEDB::HomOR(rt, r, ca, cb, cuHEDB::LOGIC, 0);
+        bad += (int32_t(phase(r.data(), s1.data(), 1024)) > 0) != bool(x | y);
+        cuHEDB::HomNOT<lvl1param>(r, ca);
+        bad += (int32_t(phase(r.data(), s1.data(), 1024)) > 0) == bool(x);
+    }
+    std::printf("dropin_demo: %s (%d mismatches)\n", bad ? "FAILED" : "ok", bad);
+    return bad ? 1 : 0;
+}

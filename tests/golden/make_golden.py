@@ -196,19 +196,21 @@ def main():
     nand = np.zeros((G, N1 + 1), np.uint32)
     for g in range(G):
         ref.ref_hom_nand_lvl1(P(nand[g]), P(ca[g]), P(cb[g]), P(bkfft), P(ksk10))
-    # MSB / identity bootstrapping of 4-bit arithmetic values scaled by 2^27
+    # MSB bootstrapping of 4-bit values filling the torus (v * 2^28), identity
+    # bootstrapping of 4-bit values under a padding bit (v * 2^27)
     vals = rng.integers(0, 16, G)
+    cm = F.tlwe_encrypt_u32(672203, 0, sk.s1, (vals << 28).astype(np.uint32), p672.alpha1)
     cv = F.tlwe_encrypt_u32(672202, 0, sk.s1, (vals << 27).astype(np.uint32), p672.alpha1)
     msb_logic = np.zeros((G, N1 + 1), np.uint32)
     msb_arith = np.zeros((G, N1 + 1), np.uint32)
     ide = np.zeros((G, N1 + 1), np.uint32)
     for g in range(G):
-        ref.ref_msb_gate_lvl1(P(msb_logic[g]), P(cv[g]), P(bkfft), P(ksk10), 1)
-        ref.ref_msb_gate_lvl1(P(msb_arith[g]), P(cv[g]), P(bkfft), P(ksk10), 0)
+        ref.ref_msb_gate_lvl1(P(msb_logic[g]), P(cm[g]), P(bkfft), P(ksk10), 1)
+        ref.ref_msb_gate_lvl1(P(msb_arith[g]), P(cm[g]), P(bkfft), P(ksk10), 0)
         ref.ref_ide_gate_lvl1(P(ide[g]), P(cv[g]), F.ctypes.c_uint32(27), P(bkfft), P(ksk10))
     np.savez_compressed(os.path.join(HERE, "gates_ref672.npz"), sk_seed=672001, bk_seed=672002,
                         ksk_seed=672003, bits=bits, ca=ca, cb=cb, nand=nand,
-                        nand_phase=F.tlwe_phase_u32(nand, sk.s1), vals=vals, cv=cv,
+                        nand_phase=F.tlwe_phase_u32(nand, sk.s1), vals=vals, cv=cv, cm=cm,
                         msb_logic=F.tlwe_phase_u32(msb_logic, sk.s1),
                         msb_arith=F.tlwe_phase_u32(msb_arith, sk.s1),
                         ide=F.tlwe_phase_u32(ide, sk.s1))

@@ -1,0 +1,218 @@
+"""GPU parity: lvl2 path, identity key switching, bootstrap operators.
+
+Bars:
+  * lvl2 TwistIFFT vs oracle: relative 1e-13; TwistFFT: within 2^16 of
+    the oracle (64-bit torus, FP64 resolves ~53 bits); round trip 2^14
+    (TFHEpp/test/polymul.cpp:58-67).
+  * lvl2 external product: within 2^32 of the oracle on uniform inputs and
+    decrypt-exact on encryptions.
+  * lvl2 PBS: decrypt-exact; phase within 2^-6 of the oracle's.
+  * IKS lvl10 / lvl20 / lvl21: bit-exact against the oracle, decrypt-exact.
+  * gates (NAND, AND, OR, NOR, XNOR, MSB, Ide): truth tables exact.
+"""
+
+import numpy as np
+import pytest
+
+import cufhedb_amd as F
+import _oracle as O
+from conftest import keys_for
+
+pytestmark = pytest.mark.gpu
+
+N2 = 2048
+D60 = 1 << 60
+MU = 1 << 29
+
+
+def z8_testvector2(lut):
+    j = np.arange(N2)
+    box = (j + 128) >> 8
+    vals = [(int(lut[b % 8]) * D60) % (1 << 64) for b in box]
+    vals = [(-v) % (1 << 64) if b == 8 else v for v, b in zip(vals, box)]
+    return np.array(vals, np.uint64)
+
+
+@pytest.fixture(scope="module")
+def lvl2(keys636):
+    k = keys636
+    bk2 = F.keygen_bk_lvl02(k.sk, 636500)
+    ctx = F.Context(k.params)
+    ctx.load_bk(2, bk2, F.BK_INT)
+    return k, bk2, ctx
+
+
+def test_gpu_fft_lvl2(lvl2, rng):
+    k, bk2, ctx = lvl2
+    a = rng.integers(0, 1 << 64, (6, N2), dtype=np.uint64)
+    fd = ctx.ifft_lvl2(a)
+    for b in range(a.shape[0]):
+        want = O.ifft64(a[b])
+        assert np.max(np.abs(fd[b] - want)) <= 1e-13 * np.max(np.abs(want))
+    back = ctx.fft_lvl2(fd)
+    for b in range(a.shape[0]):
+        assert np.max(np.abs(O.tdiff64(back[b], O.fft64(fd[b])))) <= 1 << 16
+        assert np.max(np.abs(O.tdiff64(back[b], a[b]))) <= 1 << 14
+
+
+def test_gpu_external_product_lvl2(lvl2, rng):
+    k, bk2, ctx = lvl2
+    s2 = k.sk.s2
+    msgs = rng.integers(0, 8, (3, N2)).astype(np.uint64) * np.uint64(D60)
+    trlwe = np.zeros((3, 2, N2), np.uint64)
+    F.check(F.lib().fhe_trlwe_encrypt_u64(77, 0, F._ptr(s2), F._ptr(msgs), 2.0 ** -52,
+                                          F._ptr(trlwe), 3))
+    for m in (1, 0):
+        trgsw = np.zeros((8, 2, N2), np.uint64)
+        F.check(F.lib().fhe_trgsw_encrypt_u64(78, m, F._ptr(s2), m, 2.0 ** -52, F._ptr(trgsw)))
+        fd = O.trgsw_to_fft64(trgsw)
+        got = ctx.external_product_lvl2(trlwe, fd)
+        for b in range(3):
+            want = O.external_product64(trlwe[b], fd)
+            assert np.max(np.abs(O.tdiff64(got[b], want))) <= 1 << 32
+            ph = O.trlwe_phase64(got[b], s2)
+            dec = (np.round(ph.view(np.int64) / D60).astype(np.int64)) % 16
+            assert np.array_equal(dec, (m * (msgs[b] >> np.uint64(60)).astype(np.int64)) % 16)
+
+
+def test_gpu_pbs_lvl02(lvl2, rng):
+    k, bk2, ctx = lvl2
+    B = 16
+    m = rng.integers(0, 8, B)
+    lut = rng.integers(0, 8, 8)
+    tv = z8_testvector2(lut)
+    c0 = k.enc_lvl0((m << 28).astype(np.uint32), index0=9000)
+    out = ctx.pbs_lvl02(c0, tv)
+    ph = O.tlwe_phase64(out, k.sk.s2)
+    dec = np.round(ph.view(np.int64) / D60).astype(np.int64) % 16
+    assert np.array_equal(dec, lut[m])
+    bkfft2 = O.bk_to_fft64(bk2)
+    want = O.pbs64(c0[0], bkfft2, tv)
+    dphi = int(O.tdiff64(ph[0], O.tlwe_phase64(want, k.sk.s2)[0]))
+    assert abs(dphi) <= 1 << 58  # 2^-6 of the torus
+    acc = ctx.blind_rotate_lvl02(c0[:2], tv)
+    assert np.array_equal(ctx.sample_extract_lvl2(acc, 0), out[:2])
+
+
+def test_gpu_sample_extract_lvl2_bit_exact(lvl2, rng):
+    k, bk2, ctx = lvl2
+    trlwe = rng.integers(0, 1 << 64, (3, 2, N2), dtype=np.uint64)
+    for idx in (0, 7, 2047):
+        got = ctx.sample_extract_lvl2(trlwe, idx)
+        for b in range(3):
+            assert np.array_equal(got[b], O.sample_extract64(trlwe[b], idx))
+
+
+@pytest.fixture(scope="module")
+def iks_ctx(keys636):
+    k = keys636
+    ctx = F.Context(k.params)
+    ctx.load_bk(1, k.bk, F.BK_INT)
+    ctx.load_ksk(10, k.ksk10)
+    return k, ctx
+
+
+def test_gpu_iks_lvl10_bit_exact(iks_ctx, rng):
+    k, ctx = iks_ctx
+    B = 64
+    m = (rng.integers(0, 8, B) << 28).astype(np.uint32)
+    c1 = k.enc_lvl1(m, index0=123)
+    got = ctx.iks(10, c1)
+    for b in range(8):
+        assert np.array_equal(got[b], O.iks32(c1[b], k.ksk10, k.n0))
+    e = O.tdiff32(F.tlwe_phase_u32(got, k.sk.s0), m)
+    assert np.max(np.abs(e)) < 1 << 25
+    # edge inputs: all-zero and all-ones masks
+    edge = np.zeros((2, 1025), np.uint32)
+    edge[1, :] = 0xFFFFFFFF
+    got = ctx.iks(10, edge)
+    for b in range(2):
+        assert np.array_equal(got[b], O.iks32(edge[b], k.ksk10, k.n0))
+
+
+@pytest.mark.parametrize("kind", [20, 21])
+def test_gpu_iks_from_lvl2_bit_exact(kind, keys636, rng):
+    k = keys636
+    ksk = F.keygen_ksk(k.sk, kind, 636700 + kind)
+    ctx = F.Context(k.params)
+    ctx.load_ksk(kind, ksk)
+    B = 16
+    m = (rng.integers(0, 8, B).astype(np.uint64) << np.uint64(60))
+    c2 = F.tlwe_encrypt_u64(636800, 0, k.sk.s2, m, 2.0 ** -52)
+    got = ctx.iks(kind, c2)
+    tn, t, bb = (k.n0, 2, 10) if kind == 20 else (1024, 10, 3)
+    key = k.sk.s0 if kind == 20 else k.sk.s1
+    for b in range(4):
+        assert np.array_equal(got[b], O.iks64(c2[b], ksk, tn, t, bb))
+    e = O.tdiff32(F.tlwe_phase_u32(got, key), (m >> np.uint64(32)).astype(np.uint32))
+    assert np.max(np.abs(e)) < 1 << 26
+    ctx.close()
+    del ksk
+
+
+def _enc_bits(k, bits, seed):
+    return k.enc_lvl1(np.where(bits == 1, MU, (-MU) % (1 << 32)).astype(np.uint32), seed=seed)
+
+
+def _dec_bits(k, c):
+    return (F.tlwe_phase_u32(c, k.sk.s1).view(np.int32) > 0).astype(np.int64)
+
+
+@pytest.mark.parametrize("gate,fn", [(F.GATE_NAND, lambda a, b: 1 - (a & b)),
+                                     (F.GATE_AND, lambda a, b: a & b),
+                                     (F.GATE_OR, lambda a, b: a | b),
+                                     (F.GATE_NOR, lambda a, b: 1 - (a | b)),
+                                     (F.GATE_XNOR, lambda a, b: 1 - (a ^ b))])
+def test_gpu_gates_truth_table(gate, fn, iks_ctx, rng):
+    k, ctx = iks_ctx
+    B = 64
+    a = rng.integers(0, 2, B)
+    b = rng.integers(0, 2, B)
+    ca, cb = _enc_bits(k, a, 1000 + gate), _enc_bits(k, b, 2000 + gate)
+    out = ctx.gate(gate, ca, cb, F.LOGIC)
+    assert np.array_equal(_dec_bits(k, out), fn(a, b))
+    if gate in (F.GATE_AND, F.GATE_OR):
+        # ARITHMETIC result (HEDB/utils/types.h:4-11): 0 -> 0, 1 -> 1/2
+        out = ctx.gate(gate, ca, cb, F.ARITHMETIC)
+        ph = O.tdiff32(F.tlwe_phase_u32(out, k.sk.s1), 0)
+        assert np.array_equal(np.round(ph / 2.0 ** 31).astype(np.int64) % 2, fn(a, b))
+
+
+def test_gpu_msb_and_ide_gates(iks_ctx, rng):
+    k, ctx = iks_ctx
+    B = 64
+    vals = rng.integers(0, 16, B)
+    # MSB of 4-bit values filling the torus (v * 2^28)
+    cm = k.enc_lvl1((vals << 28).astype(np.uint32), seed=3001)
+    msb = (vals >> 3) & 1
+    out = ctx.msb_gate(cm, F.LOGIC)
+    assert np.array_equal(_dec_bits(k, out), msb)
+    out = ctx.msb_gate(cm, F.ARITHMETIC)  # 1 -> 1/2, 0 -> 0
+    ph = O.tdiff32(F.tlwe_phase_u32(out, k.sk.s1), 0)
+    assert np.array_equal(np.round(ph / 2.0 ** 31).astype(np.int64) % 2, msb)
+    # identity on 4-bit values under a padding bit (v * 2^27), output scale 2^27
+    cv = k.enc_lvl1((vals << 27).astype(np.uint32), seed=3000)
+    out = ctx.ide_gate(cv, 27)
+    ph = O.tdiff32(F.tlwe_phase_u32(out, k.sk.s1), 0)
+    assert np.array_equal(np.round(ph / (1 << 27)).astype(np.int64) % 32, vals)
+
+
+def test_gpu_gates_match_reference_golden(rng):
+    """Same inputs and keys as the reference's HomNAND / MSB / Ide vectors."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "gates_ref672.npz"))
+    k = keys_for(F.PRESET_REF672)
+    assert int(g["sk_seed"]) == k.seed and int(g["bk_seed"]) == k.seed + 1
+    ctx = F.Context(k.params)
+    ctx.load_bk(1, k.bk, F.BK_INT)
+    ctx.load_ksk(10, F.keygen_ksk(k.sk, 10, int(g["ksk_seed"])))
+    tol = 1 << 26
+    out = ctx.gate(F.GATE_NAND, g["ca"], g["cb"], F.LOGIC)
+    ph = F.tlwe_phase_u32(out, k.sk.s1)
+    assert np.max(np.abs(O.tdiff32(ph, g["nand_phase"]))) <= tol
+    for name, rt in (("msb_logic", F.LOGIC), ("msb_arith", F.ARITHMETIC)):
+        ph = F.tlwe_phase_u32(ctx.msb_gate(g["cm"], rt), k.sk.s1)
+        assert np.max(np.abs(O.tdiff32(ph, g[name]))) <= tol
+    ph = F.tlwe_phase_u32(ctx.ide_gate(g["cv"], 27), k.sk.s1)
+    assert np.max(np.abs(O.tdiff32(ph, g["ide"]))) <= tol
+    ctx.close()

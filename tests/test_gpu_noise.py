@@ -1,0 +1,50 @@
+"""GPU vs oracle output-noise statistics for the lvl1 PBS (n0 = 636 set).
+
+A per-sample phase comparison between two FP64 implementations is only
+meaningful up to the noise (they accumulate independent rounding noise once
+a low gadget digit flips), so this checks the distribution: over B outputs
+the GPU's phase-error std must match the oracle's (computed on the same keys
+and a subsample of the same inputs) within the statistical error, its mean
+must be centred, and no output may decrypt wrongly.  A lost bit of FFT
+precision, a wrong rounding offset or a biased truncation would move one of
+these by many standard errors.
+"""
+
+import concurrent.futures as cf
+import os
+
+import numpy as np
+import pytest
+
+import cufhedb_amd as F
+import _oracle as O
+from test_gpu_lvl1 import z8_testvector
+
+pytestmark = pytest.mark.gpu
+
+
+def test_gpu_pbs_noise_matches_oracle(keys636, ctx636):
+    k = keys636
+    rng = np.random.default_rng(31337)
+    B = 4096
+    m = rng.integers(0, 8, B)
+    lut = rng.integers(0, 8, 8)
+    tv = z8_testvector(lut)
+    c0 = k.enc_lvl0((m << 28).astype(np.uint32), index0=50000)
+    out = ctx636.pbs_lvl01(c0, tv)
+    want = (lut[m] << 28).astype(np.uint32)
+    e_gpu = O.tdiff32(F.tlwe_phase_u32(out, k.sk.s1), want)
+    assert np.max(np.abs(e_gpu)) < 1 << 27  # no decryption failure
+    S = 96
+    threads = min(16, os.cpu_count() or 1)
+    bkfft = k.bkfft
+    with cf.ThreadPoolExecutor(threads) as ex:
+        outs = list(ex.map(lambda b: O.pbs32(c0[b], bkfft, tv), range(S)))
+    e_orc = O.tdiff32(F.tlwe_phase_u32(np.stack(outs), k.sk.s1), want[:S])
+    s_gpu, s_orc = e_gpu.std(), e_orc.std()
+    ratio = s_gpu / s_orc
+    print(f"phase-error std: gpu 2^{np.log2(s_gpu):.2f} over {B}, "
+          f"oracle 2^{np.log2(s_orc):.2f} over {S}, ratio {ratio:.3f}, "
+          f"gpu mean {e_gpu.mean():.0f}")
+    assert 0.78 < ratio < 1.28  # ~3 standard errors of a 96-sample std
+    assert abs(e_gpu.mean()) < 4 * s_gpu / np.sqrt(B)
