@@ -112,6 +112,10 @@ _SIGS = {
     "fhe_gate_lvl1_dev": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp]),
     "fhe_msb_gate_lvl1_dev": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
     "fhe_ide_gate_lvl1_dev": (_i, [_vp, _vp, _vp, _u32, _i, _vp]),
+    "fhe_msb_gate_dev": (_i, [_vp, _i, _i, _vp, _vp, _i, _i, _vp]),
+    "fhe_ide_gate_dev": (_i, [_vp, _i, _i, _vp, _vp, _u32, _i, _vp]),
+    "fhe_log_to_ari_lvl1_dev": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "fhe_log_rescale_lvl1_dev": (_i, [_vp, _vp, _vp, _u32, _i, _vp]),
     "fhe_pbs_lvl01": (_i, [_vp, _vp, _vp, _i, _vp, _i]),
     "fhe_iks": (_i, [_vp, _i, _vp, _vp, _i]),
     "fhe_malloc": (_i, [_vp, _sz]),
@@ -453,6 +457,39 @@ class Context:
         a = np.ascontiguousarray(a, np.uint32)
         return self._run_dev(lib().fhe_ide_gate_lvl1_dev, [a], a.shape, np.uint32, scale_bits,
                              a.shape[0])
+
+
+def _level_op(ctx, fn, in_level, out_level, a, arg):
+    a = np.ascontiguousarray(a, np.uint32 if in_level == 1 else np.uint64)
+    B = a.shape[0]
+    N = LVL1_N if out_level == 1 else LVL2_N
+    dt = np.uint32 if out_level == 1 else np.uint64
+    din = DeviceBuffer.from_array(a)
+    dout = DeviceBuffer(B * (N + 1) * np.dtype(dt).itemsize)
+    check(fn(ctx.h, in_level, out_level, din.ptr, dout.ptr, arg, B, None))
+    ctx.sync()
+    return dout.to_array(dt, (B, N + 1))
+
+
+def msb_gate_levels(ctx, in_level, out_level, a, result_type=LOGIC):
+    """cuHEDB::MSBGateBootstrapping for (1,1), (2,1) or (2,2)."""
+    return _level_op(ctx, lib().fhe_msb_gate_dev, in_level, out_level, a, result_type)
+
+
+def ide_gate_levels(ctx, in_level, out_level, a, scale_bits):
+    """cuHEDB::IdeGateBootstrapping for (1,1), (2,1) or (2,2)."""
+    return _level_op(ctx, lib().fhe_ide_gate_dev, in_level, out_level, a, scale_bits)
+
+
+def log_to_ari(ctx, a):
+    a = np.ascontiguousarray(a, np.uint32)
+    return ctx._run_dev(lib().fhe_log_to_ari_lvl1_dev, [a], a.shape, np.uint32, a.shape[0])
+
+
+def log_rescale(ctx, a, scale_bits):
+    a = np.ascontiguousarray(a, np.uint32)
+    return ctx._run_dev(lib().fhe_log_rescale_lvl1_dev, [a], a.shape, np.uint32, scale_bits,
+                        a.shape[0])
 
 
 def device_name():

@@ -31,6 +31,9 @@ struct fhe_ctx {
     int variant = FHE_PBS_FP64;
     // grow-on-demand scratch for the bootstrap operators
     uint32_t *tv1 = nullptr;     // one lvl1 test vector
+    uint64_t *tv2 = nullptr;     // one lvl2 test vector
+    uint32_t *tmp2 = nullptr;    // B x 2049 uint64 (as 2 x uint32 words)
+    size_t tmp2_cap = 0;
     uint32_t *tmp1 = nullptr;    // B x 1025
     size_t tmp1_cap = 0;
     uint32_t *tmp0 = nullptr;    // B x (n0 + 1)
@@ -93,6 +96,7 @@ int fhe_ctx_create(const fhe_params *p, int device, fhe_ctx **out)
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
     if (e == hipSuccess) e = hipMalloc(&ctx->tv1, Lvl1::N * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&ctx->tv2, Lvl2::N * sizeof(uint64_t));
     if (e != hipSuccess) {
         fhe_ctx_destroy(ctx);
         return hip_fail(e, "fhe_ctx_create");
@@ -107,7 +111,7 @@ int fhe_ctx_destroy(fhe_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     void *bufs[] = {ctx->bk1, ctx->bk2, ctx->ksk[0], ctx->ksk[1], ctx->ksk[2], ctx->trgsw1,
-                    ctx->trgsw2, ctx->tv1, ctx->tmp1, ctx->tmp0};
+                    ctx->trgsw2, ctx->tv1, ctx->tv2, ctx->tmp1, ctx->tmp0, ctx->tmp2};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -345,30 +349,93 @@ int fhe_fft_lvl2_dev(fhe_ctx *ctx, const double *fd_dev, uint64_t *poly_dev, int
 // ---- bootstrap operators --------------------------------------------------
 
 namespace {
-// c = sa*a + sb*b + off -> IKS lvl10 -> PBS(tv = const u) [-> body += post]
+
+// Test vector of a bootstrap operator: constant u, or identity
+// 2^scale * (i >> 6) (gpolygen, src/HEDB/comparison/tfhepp_utils.h:113-120).
+struct TvSpec {
+    bool ide;
+    uint64_t value;  // u, or scale_bits
+};
+
+// in (lvl `in_level` TLWEs already offset, device) -> IKS to lvl0 -> PBS to
+// lvl `out_level` with the test vector -> body += post.
+int run_bootstrap(fhe_ctx *ctx, int in_level, const void *in, int out_level, void *out, TvSpec tv,
+                  uint64_t post, int B, hipStream_t s)
+{
+    const int kind = in_level == 1 ? 10 : 20;
+    const int slot = ksk_slot(kind);
+    cd *bk = out_level == 1 ? ctx->bk1 : ctx->bk2;
+    if (!bk || !ctx->ksk[slot]) {
+        set_error("bootstrap operator: lvl0%d BK and KSK %d must be loaded", out_level, kind);
+        return FHE_ERR_NOT_LOADED;
+    }
+    if (B == 0) return FHE_OK;
+    int rc = grow(ctx->tmp0, ctx->tmp0_cap, size_t(B) * (ctx->p.n0 + 1));
+    if (rc) return rc;
+    const int n0 = int(ctx->p.n0);
+    HIP_TRY(launch_iks(kind, in, ctx->ksk[slot], ctx->tmp0, n0, B, s), "IKS");
+    if (out_level == 1) {
+        HIP_TRY(tv.ide ? launch_tv_ide<uint32_t>(ctx->tv1, uint32_t(tv.value), Lvl1::N, s)
+                       : launch_tv_const<uint32_t>(ctx->tv1, uint32_t(tv.value), Lvl1::N, s),
+                "test vector");
+        HIP_TRY(launch_blind_rotate1(ctx->tmp0, n0, ctx->bk1, ctx->tv1, 0,
+                                     static_cast<uint32_t *>(out), true, B, s),
+                "pbs lvl01");
+        if (post)
+            HIP_TRY(launch_tlwe_add_body<uint32_t>(static_cast<uint32_t *>(out), Lvl1::N + 1,
+                                                   uint32_t(post), B, s),
+                    "post offset");
+    } else {
+        HIP_TRY(tv.ide ? launch_tv_ide<uint64_t>(ctx->tv2, uint32_t(tv.value), Lvl2::N, s)
+                       : launch_tv_const<uint64_t>(ctx->tv2, tv.value, Lvl2::N, s),
+                "test vector");
+        HIP_TRY(launch_blind_rotate2(ctx->tmp0, n0, ctx->bk2, ctx->tv2, 0,
+                                     static_cast<uint64_t *>(out), true, B, s),
+                "pbs lvl02");
+        if (post)
+            HIP_TRY(launch_tlwe_add_body<uint64_t>(static_cast<uint64_t *>(out), Lvl2::N + 1, post,
+                                                   B, s),
+                    "post offset");
+    }
+    return FHE_OK;
+}
+
+// in + pre (on the body) into level-matched scratch; returns the scratch
+int offset_input(fhe_ctx *ctx, int in_level, const void *in, uint64_t pre, int B, hipStream_t s,
+                 const void **staged)
+{
+    if (in_level == 1) {
+        int rc = grow(ctx->tmp1, ctx->tmp1_cap, size_t(B) * (Lvl1::N + 1));
+        if (rc) return rc;
+        HIP_TRY(launch_tlwe_copy_add_body<uint32_t>(static_cast<const uint32_t *>(in), ctx->tmp1,
+                                                    Lvl1::N + 1, uint32_t(pre), B, s),
+                "input offset");
+        *staged = ctx->tmp1;
+    } else {
+        int rc = grow(ctx->tmp2, ctx->tmp2_cap, size_t(B) * (Lvl2::N + 1) * 2);
+        if (rc) return rc;
+        uint64_t *t2 = reinterpret_cast<uint64_t *>(ctx->tmp2);
+        HIP_TRY(launch_tlwe_copy_add_body<uint64_t>(static_cast<const uint64_t *>(in), t2,
+                                                    Lvl2::N + 1, pre, B, s),
+                "input offset");
+        *staged = t2;
+    }
+    return FHE_OK;
+}
+
+// c = sa*a + sb*b + off -> IKS lvl10 -> PBS(tv) [-> body += post]
 int gate_pipeline(fhe_ctx *ctx, const uint32_t *a, const uint32_t *b, int sa, int sb, uint32_t off,
                   uint32_t u, bool ide, uint32_t scale_bits, uint32_t post, uint32_t *out, int B,
                   hipStream_t s)
 {
-    if (!ctx->bk1 || !ctx->ksk[0]) {
-        set_error("bootstrap operator: lvl01 BK and KSK 10 must be loaded");
-        return FHE_ERR_NOT_LOADED;
-    }
     if (B == 0) return FHE_OK;
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     int rc = grow(ctx->tmp1, ctx->tmp1_cap, size_t(B) * (Lvl1::N + 1));
     if (rc) return rc;
-    rc = grow(ctx->tmp0, ctx->tmp0_cap, size_t(B) * (ctx->p.n0 + 1));
-    if (rc) return rc;
     HIP_TRY(launch_tlwe1_lincomb(a, b ? b : a, sa, b ? sb : 0, off, ctx->tmp1, B, s), "lincomb");
-    HIP_TRY(launch_iks(10, ctx->tmp1, ctx->ksk[0], ctx->tmp0, int(ctx->p.n0), B, s), "IKS lvl10");
-    HIP_TRY(ide ? launch_tv1_ide(ctx->tv1, scale_bits, s) : launch_tv1_const(ctx->tv1, u, s),
-            "test vector");
-    HIP_TRY(launch_blind_rotate1(ctx->tmp0, int(ctx->p.n0), ctx->bk1, ctx->tv1, 0, out, true, B, s),
-            "pbs lvl01");
-    if (post) HIP_TRY(launch_tlwe1_add_body(out, post, B, s), "post offset");
-    return FHE_OK;
+    return run_bootstrap(ctx, 1, ctx->tmp1, 1, out, TvSpec{ide, ide ? scale_bits : u}, post, B, s);
 }
+
 }  // namespace
 
 int fhe_gate_lvl1_dev(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint32_t *b_dev,
@@ -418,6 +485,75 @@ int fhe_ide_gate_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_de
     FHE_CHECK_ARG(scale_bits < 32, "fhe_ide_gate_lvl1_dev: scale_bits must be < 32");
     // cuHEDB::IdeGateBootstrapping: +2^26 offset, tv = gpolygen(4, scale_bits)
     return gate_pipeline(ctx, in_dev, nullptr, 1, 0, 1u << 26, 0, true, scale_bits, 0, out_dev, B,
+                         pick(ctx, stream));
+}
+
+int fhe_msb_gate_dev(fhe_ctx *ctx, int in_level, int out_level, const void *in_dev, void *out_dev,
+                     int result_type, int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && in_dev && out_dev && B >= 0, "fhe_msb_gate_dev: bad argument");
+    FHE_CHECK_ARG((in_level == 1 || in_level == 2) && (out_level == 1 || out_level == 2) &&
+                      !(in_level == 1 && out_level == 2),
+                  "fhe_msb_gate_dev: levels (1,1), (2,1) or (2,2)");
+    if (B == 0) return FHE_OK;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    hipStream_t s = pick(ctx, stream);
+    // gatebootstrap_gpu.cuh:31-80: offset 2^(w-6) (lvl1 output) / 2^(w-7)
+    // (lvl2 output), w = input torus width; tv = -u; ARITHMETIC adds u
+    const int w = in_level == 1 ? 32 : 64;
+    const uint64_t pre = uint64_t(1) << (w - (out_level == 1 ? 6 : 7));
+    const uint64_t mu = out_level == 1 ? uint64_t(Lvl1::mu) : Lvl2::mu;
+    const uint64_t u = result_type ? mu : mu << 1;
+    const uint64_t mask = out_level == 1 ? 0xFFFFFFFFull : ~0ull;
+    const void *staged = nullptr;
+    int rc = offset_input(ctx, in_level, in_dev, pre, B, s, &staged);
+    if (rc) return rc;
+    return run_bootstrap(ctx, in_level, staged, out_level, out_dev, TvSpec{false, (0 - u) & mask},
+                         result_type ? 0 : u, B, s);
+}
+
+int fhe_ide_gate_dev(fhe_ctx *ctx, int in_level, int out_level, const void *in_dev, void *out_dev,
+                     uint32_t scale_bits, int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && in_dev && out_dev && B >= 0, "fhe_ide_gate_dev: bad argument");
+    FHE_CHECK_ARG((in_level == 1 || in_level == 2) && (out_level == 1 || out_level == 2) &&
+                      !(in_level == 1 && out_level == 2),
+                  "fhe_ide_gate_dev: levels (1,1), (2,1) or (2,2)");
+    // gatebootstrap_gpu.cuh:82-115: lvl2 -> lvl1 uses scale_bits - 32
+    uint32_t sb = scale_bits;
+    if (in_level == 2 && out_level == 1) {
+        FHE_CHECK_ARG(scale_bits >= 32 && scale_bits < 64, "fhe_ide_gate_dev: scale_bits in [32, 64)");
+        sb = scale_bits - 32;
+    }
+    FHE_CHECK_ARG(sb < (out_level == 1 ? 32u : 64u), "fhe_ide_gate_dev: scale_bits too large");
+    if (B == 0) return FHE_OK;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    hipStream_t s = pick(ctx, stream);
+    const int w = in_level == 1 ? 32 : 64;
+    const uint64_t pre = uint64_t(1) << (w - (out_level == 1 ? 6 : 7));
+    const void *staged = nullptr;
+    int rc = offset_input(ctx, in_level, in_dev, pre, B, s, &staged);
+    if (rc) return rc;
+    return run_bootstrap(ctx, in_level, staged, out_level, out_dev, TvSpec{true, sb}, 0, B, s);
+}
+
+int fhe_log_to_ari_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_dev, int B,
+                            void *stream)
+{
+    FHE_CHECK_ARG(ctx && in_dev && out_dev && B >= 0, "fhe_log_to_ari_lvl1_dev: bad argument");
+    // gatebootstrap_gpu.cuh:155-162: tv = -2 mu, body += mu
+    return gate_pipeline(ctx, in_dev, nullptr, 1, 0, 0, 0u - (Lvl1::mu << 1), false, 0, Lvl1::mu,
+                         out_dev, B, pick(ctx, stream));
+}
+
+int fhe_log_rescale_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_dev,
+                             uint32_t scale_bits, int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && in_dev && out_dev && B >= 0, "fhe_log_rescale_lvl1_dev: bad argument");
+    FHE_CHECK_ARG(scale_bits >= 1 && scale_bits <= 32, "fhe_log_rescale_lvl1_dev: scale_bits in [1, 32]");
+    // gatebootstrap_gpu.cuh:164-170: mu' = 2^(scale_bits-1), tv = -mu', body += mu'
+    const uint32_t m = uint32_t(1ull << (scale_bits - 1));
+    return gate_pipeline(ctx, in_dev, nullptr, 1, 0, 0, 0u - m, false, 0, m, out_dev, B,
                          pick(ctx, stream));
 }
 

@@ -147,6 +147,21 @@ __device__ __forceinline__ Twiddles1024 make_twiddles512_plain(int lane)
     return t;
 }
 
+// Twiddle bases kept in LDS (3 x 64 complex per wave) instead of registers.
+struct TwLds {
+    const cd *t;  // [3][64]: wl, w4l, w64
+    __device__ __forceinline__ Twiddles1024 get(int lane) const
+    {
+        return Twiddles1024{t[lane], t[64 + lane], t[128 + lane]};
+    }
+};
+__device__ __forceinline__ void store_twiddles(cd *t, int lane, const Twiddles1024 &tw)
+{
+    t[lane] = tw.wl;
+    t[64 + lane] = tw.w4l;
+    t[128 + lane] = tw.w64;
+}
+
 constexpr int kFftPad = 65;                 // LDS row stride in double2
 constexpr int kFftLdsElems = 8 * kFftPad;   // 520 double2 = 8320 bytes
 
@@ -263,14 +278,14 @@ __host__ __device__ constexpr int rev9(int x)
     return r;
 }
 
-// uint32_t(int64_t(x)) for |x| < 2^53 (TwistFFT truncation,
-// fft_processor_spqlios.cpp:121), computed exactly in FP64.
+// uint32_t(int64_t(x)) for |x| < 2^51 (TwistFFT truncation,
+// fft_processor_spqlios.cpp:121), computed exactly in FP64: t = trunc(x) is
+// an integer of magnitude < 2^51, so t + 1.5*2^52 is exact and its low 32
+// mantissa bits are t mod 2^32 (two's complement for negative t).
 __device__ __forceinline__ uint32_t trunc_torus32(double x)
 {
-    double t = trunc(x);
-    double q = floor(t * 0x1p-32);
-    double r = __fma_rn(-q, 0x1p32, t);
-    return uint32_t(r);
+    const double u = trunc(x) + 0x1.8p52;
+    return uint32_t(__double_as_longlong(u));
 }
 
 }  // namespace fhe

@@ -88,7 +88,85 @@ __global__ void k_tv1_ide(uint32_t *tv, uint32_t scale_bits)
     if (i < Lvl1::N) tv[i] = (1u << scale_bits) * uint32_t(i >> 6);
 }
 
+// out = in with `add` added to the body word (W words per TLWE)
+template <class T>
+__global__ void k_tlwe_copy_add_body(const T *__restrict__ in, T *__restrict__ out, int W, T add,
+                                     int B)
+{
+    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= size_t(B) * W) return;
+    T v = in[gid];
+    if (gid % W == size_t(W - 1)) v += add;
+    out[gid] = v;
+}
+
+template <class T>
+__global__ void k_tlwe_add_body(T *__restrict__ tlwe, int W, T add, int B)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < B) tlwe[size_t(b) * W + W - 1] += add;
+}
+
+template <class T>
+__global__ void k_tv_const(T *tv, T u, int N)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) tv[i] = u;
+}
+
+// gpolygen with padding 6: 2^scale * (i >> 6)
+template <class T>
+__global__ void k_tv_ide(T *tv, uint32_t scale_bits, int N)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) tv[i] = (T(1) << scale_bits) * T(i >> 6);
+}
+
 }  // namespace
+
+template <class T>
+hipError_t launch_tlwe_copy_add_body(const T *in, T *out, int W, T add, int B, hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    const size_t n = size_t(B) * W;
+    hipLaunchKernelGGL(k_tlwe_copy_add_body<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s,
+                       in, out, W, add, B);
+    return hipGetLastError();
+}
+template hipError_t launch_tlwe_copy_add_body<uint32_t>(const uint32_t *, uint32_t *, int, uint32_t,
+                                                        int, hipStream_t);
+template hipError_t launch_tlwe_copy_add_body<uint64_t>(const uint64_t *, uint64_t *, int, uint64_t,
+                                                        int, hipStream_t);
+
+template <class T>
+hipError_t launch_tlwe_add_body(T *tlwe, int W, T add, int B, hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tlwe_add_body<T>, dim3(unsigned((B + 255) / 256)), dim3(256), 0, s, tlwe,
+                       W, add, B);
+    return hipGetLastError();
+}
+template hipError_t launch_tlwe_add_body<uint32_t>(uint32_t *, int, uint32_t, int, hipStream_t);
+template hipError_t launch_tlwe_add_body<uint64_t>(uint64_t *, int, uint64_t, int, hipStream_t);
+
+template <class T>
+hipError_t launch_tv_const(T *tv, T u, int N, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tv_const<T>, dim3(unsigned((N + 255) / 256)), dim3(256), 0, s, tv, u, N);
+    return hipGetLastError();
+}
+template hipError_t launch_tv_const<uint32_t>(uint32_t *, uint32_t, int, hipStream_t);
+template hipError_t launch_tv_const<uint64_t>(uint64_t *, uint64_t, int, hipStream_t);
+
+template <class T>
+hipError_t launch_tv_ide(T *tv, uint32_t scale_bits, int N, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tv_ide<T>, dim3(unsigned((N + 255) / 256)), dim3(256), 0, s, tv,
+                       scale_bits, N);
+    return hipGetLastError();
+}
+template hipError_t launch_tv_ide<uint32_t>(uint32_t *, uint32_t, int, hipStream_t);
+template hipError_t launch_tv_ide<uint64_t>(uint64_t *, uint32_t, int, hipStream_t);
 
 hipError_t launch_iks(int kind, const void *in, const uint32_t *ksk, uint32_t *out, int tn, int B,
                       hipStream_t s)

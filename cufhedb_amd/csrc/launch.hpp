@@ -54,4 +54,14 @@ hipError_t launch_tlwe1_add_body(uint32_t *tlwe, uint32_t add, int B, hipStream_
 hipError_t launch_tv1_const(uint32_t *tv, uint32_t u, hipStream_t s);
 hipError_t launch_tv1_ide(uint32_t *tv, uint32_t scale_bits, hipStream_t s);
 
+// generic forms for either torus width (W words per TLWE)
+template <class T>
+hipError_t launch_tlwe_copy_add_body(const T *in, T *out, int W, T add, int B, hipStream_t s);
+template <class T>
+hipError_t launch_tlwe_add_body(T *tlwe, int W, T add, int B, hipStream_t s);
+template <class T>
+hipError_t launch_tv_const(T *tv, T u, int N, hipStream_t s);
+template <class T>
+hipError_t launch_tv_ide(T *tv, uint32_t scale_bits, int N, hipStream_t s);
+
 }  // namespace fhe

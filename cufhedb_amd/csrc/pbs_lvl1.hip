@@ -27,6 +27,9 @@ namespace {
 constexpr int N1 = Lvl1::N;
 constexpr uint32_t kDecOff1 = decomp_total_offset<uint32_t, 32, Lvl1::l, Lvl1::Bgbit>();
 constexpr int kPolyCd = 512;  // complex points per polynomial
+// Device-layout keys hold FD / 512: the inverse transform's 2/N scale
+// (fft_processor_spqlios.cpp:97-117) is folded into the key, exactly.
+constexpr double kKeyScale1 = 1.0 / 512;
 
 // (X^a - 1) * poly at coefficient n, a in [0, 2N)
 __device__ __forceinline__ uint32_t rot_minus_one(const uint32_t *poly, uint32_t a, int n)
@@ -37,13 +40,20 @@ __device__ __forceinline__ uint32_t rot_minus_one(const uint32_t *poly, uint32_t
     return v - poly[n];
 }
 
-// signed gadget digit d of x (DecompositionPolynomial), as a double
-__device__ __forceinline__ double digit1(uint32_t x, int d)
+// Gadget digits of x (DecompositionPolynomial, trgsw.cpp:21-39):
+//   digit_d = ((x + offset) >> (32 - 6(d+1)) & 63) - 32.
+// prep1 adds the offset and flips the top bit of every 6-bit field, so each
+// digit is then one signed 6-bit field extract: (f ^ 32) read as a signed
+// 6-bit number equals f - 32.
+constexpr uint32_t kDigitFlip1 = (1u << 31) | (1u << 25) | (1u << 19);
+__device__ __forceinline__ uint32_t prep1(uint32_t x) { return (x + kDecOff1) ^ kDigitFlip1; }
+__device__ __forceinline__ double digit1p(uint32_t u, int d)
 {
-    const int sh = 32 - Lvl1::Bgbit * (d + 1);
-    return double(int32_t((x + kDecOff1) >> sh & ((1u << Lvl1::Bgbit) - 1)) -
-                  (1 << (Lvl1::Bgbit - 1)));
+    // field [sh, sh + 6) moved to the top, then arithmetic shift down
+    const int up = Lvl1::Bgbit * d;  // = 32 - 6 - sh
+    return double(int32_t(u << up) >> (32 - Lvl1::Bgbit));
 }
+__device__ __forceinline__ double digit1(uint32_t x, int d) { return digit1p(prep1(x), d); }
 
 // load one TRGSW row (both output columns) of the device key layout
 __device__ __forceinline__ void load_row(const cd *row, int lane, cd b0[8], cd b1[8])
@@ -63,12 +73,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 {
     __shared__ uint32_t acc[2 * N1];
     __shared__ cd tile[kFftLdsElems];
+    __shared__ cd twl[3 * 64];
     const int lane = threadIdx.x;
     const int ct = blockIdx.x;
     if (ct >= B) return;
     const uint32_t *c0 = in + size_t(ct) * (n0 + 1);
     const uint32_t *tvp = tv + size_t(ct) * tv_stride;
-    const Twiddles1024 tw = make_twiddles1024(lane);
+    store_twiddles(twl, lane, make_twiddles1024(lane));
+    const TwLds twp{twl};
 
     // acc = (0, X^bbar * tv), bbar = 2N - (b >> 21)   (no rounding)
     const uint32_t bbar = 2u * N1 - (c0[n0] >> (32 - 1 - Lvl1::nbit));
@@ -91,40 +103,47 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 #pragma unroll
         for (int q = 0; q < 8; q++) fd0[q] = fd1[q] = cd{0.0, 0.0};
 #pragma unroll 1
-        for (int r = 0; r < Lvl1::rows; r++) {
-            const int c = r / Lvl1::l, d = r - c * Lvl1::l;
+        for (int c = 0; c < 2; c++) {
+            // (X^abar - 1) acc_c, offset and flipped for digit extraction,
+            // read once per polynomial from LDS
             const uint32_t *poly = acc + c * N1;
-            // issue this row's key loads before the FFT so their latency
-            // overlaps the transform
-            cd b0[8], b1[8];
-            load_row(bki + r * 2 * kPolyCd, lane, b0, b1);
-            // digits of (X^abar - 1) acc_c, recomputed per digit from LDS
-            cd v[8];
+            uint32_t t[16];
 #pragma unroll
-            for (int j = 0; j < 8; j++)
-                v[j] = cd{digit1(rot_minus_one(poly, abar, lane + 64 * j), d),
-                          digit1(rot_minus_one(poly, abar, lane + 64 * j + 512), d)};
-            fft512_fwd(v, tile, lane, tw);
+            for (int j = 0; j < 8; j++) {
+                t[2 * j] = prep1(rot_minus_one(poly, abar, lane + 64 * j));
+                t[2 * j + 1] = prep1(rot_minus_one(poly, abar, lane + 64 * j + 512));
+            }
+#pragma unroll 1
+            for (int d = 0; d < Lvl1::l; d++) {
+                // issue this row's key loads before the FFT so their latency
+                // overlaps the transform
+                cd b0[8], b1[8];
+                load_row(bki + (c * Lvl1::l + d) * 2 * kPolyCd, lane, b0, b1);
+                cd v[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                cmac(fd0[q], v[q], b0[q]);
-                cmac(fd1[q], v[q], b1[q]);
+                for (int j = 0; j < 8; j++) v[j] = cd{digit1p(t[2 * j], d), digit1p(t[2 * j + 1], d)};
+                fft512_fwd(v, tile, lane, twp.get(lane));
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    cmac(fd0[q], v[q], b0[q]);
+                    cmac(fd1[q], v[q], b1[q]);
+                }
             }
         }
         // every lane's rotation reads are complete (the FFTs synchronised)
-        fft512_inv(fd0, tile, lane, tw);
+        fft512_inv(fd0, tile, lane, twp.get(lane));
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int n = lane + 64 * j;
-            acc[n] += trunc_torus32(fd0[j].x * (1.0 / 512));
-            acc[n + 512] += trunc_torus32(fd0[j].y * (1.0 / 512));
+            acc[n] += trunc_torus32(fd0[j].x);
+            acc[n + 512] += trunc_torus32(fd0[j].y);
         }
-        fft512_inv(fd1, tile, lane, tw);
+        fft512_inv(fd1, tile, lane, twp.get(lane));
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int n = lane + 64 * j;
-            acc[N1 + n] += trunc_torus32(fd1[j].x * (1.0 / 512));
-            acc[N1 + n + 512] += trunc_torus32(fd1[j].y * (1.0 / 512));
+            acc[N1 + n] += trunc_torus32(fd1[j].x);
+            acc[N1 + n + 512] += trunc_torus32(fd1[j].y);
         }
         __syncthreads();
     }
@@ -183,14 +202,14 @@ __global__ __launch_bounds__(64) void k_extprod1(const uint32_t *__restrict__ tr
     fft512_inv(fd0, tile, lane, tw);
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-        o[lane + 64 * j] = trunc_torus32(fd0[j].x * (1.0 / 512));
-        o[lane + 64 * j + 512] = trunc_torus32(fd0[j].y * (1.0 / 512));
+        o[lane + 64 * j] = trunc_torus32(fd0[j].x);
+        o[lane + 64 * j + 512] = trunc_torus32(fd0[j].y);
     }
     fft512_inv(fd1, tile, lane, tw);
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-        o[N1 + lane + 64 * j] = trunc_torus32(fd1[j].x * (1.0 / 512));
-        o[N1 + lane + 64 * j + 512] = trunc_torus32(fd1[j].y * (1.0 / 512));
+        o[N1 + lane + 64 * j] = trunc_torus32(fd1[j].x);
+        o[N1 + lane + 64 * j + 512] = trunc_torus32(fd1[j].y);
     }
 }
 
@@ -210,7 +229,8 @@ __global__ __launch_bounds__(64) void k_poly_to_devfd1(const uint32_t *__restric
         v[j] = cd{double(int32_t(a[lane + 64 * j])), double(int32_t(a[lane + 64 * j + 512]))};
     fft512_fwd(v, tile, lane, tw);
 #pragma unroll
-    for (int q = 0; q < 8; q++) fd[size_t(p) * kPolyCd + q * 64 + lane] = v[q];
+    for (int q = 0; q < 8; q++)
+        fd[size_t(p) * kPolyCd + q * 64 + lane] = cd{v[q].x * kKeyScale1, v[q].y * kKeyScale1};
 }
 
 // spqlios-order FD polynomials -> device layout (pure permutation)
@@ -222,7 +242,7 @@ __global__ void k_spq_to_devfd1(const double *__restrict__ spq, cd *__restrict__
     const int e = int(gid % kPolyCd);
     const int q = e >> 6, s = e & 63;
     const int idx = rev9(fd_index1024(s, q));
-    fd[gid] = cd{spq[p * N1 + idx], spq[p * N1 + idx + 512]};
+    fd[gid] = cd{spq[p * N1 + idx] * kKeyScale1, spq[p * N1 + idx + 512] * kKeyScale1};
 }
 
 // torus polynomial -> spqlios-order FD (TwistIFFT)
@@ -269,8 +289,8 @@ __global__ __launch_bounds__(64) void k_fft1(const double *__restrict__ fd,
     uint32_t *o = poly + size_t(p) * N1;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-        o[lane + 64 * j] = trunc_torus32(v[j].x * (1.0 / 512));
-        o[lane + 64 * j + 512] = trunc_torus32(v[j].y * (1.0 / 512));
+        o[lane + 64 * j] = trunc_torus32(v[j].x * kKeyScale1);
+        o[lane + 64 * j + 512] = trunc_torus32(v[j].y * kKeyScale1);
     }
 }
 

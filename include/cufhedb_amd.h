@@ -200,6 +200,22 @@ int fhe_msb_gate_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_de
 int fhe_ide_gate_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_dev,
                           uint32_t scale_bits, int B, void *stream);
 
+/* Level-generic forms (gatebootstrap_gpu.cuh:31-115): (in_level, out_level)
+ * = (1,1), (2,1) through IKS lvl20 + PBS lvl01, or (2,2) through IKS lvl20 +
+ * PBS lvl02.  Inputs / outputs are TLWEs of their level (uint32_t for lvl1,
+ * uint64_t for lvl2).  For (2,1) identity bootstrapping scale_bits refers to
+ * the 64-bit torus, as in the reference (it applies scale_bits - 32). */
+int fhe_msb_gate_dev(fhe_ctx *ctx, int in_level, int out_level, const void *in_dev, void *out_dev,
+                     int result_type, int B, void *stream);
+int fhe_ide_gate_dev(fhe_ctx *ctx, int in_level, int out_level, const void *in_dev, void *out_dev,
+                     uint32_t scale_bits, int B, void *stream);
+/* cuHEDB::LOG_to_ARI (gatebootstrap_gpu.cuh:155-162) and cuHEDB::log_rescale
+ * (:164-170): IKS lvl10, PBS with a constant test vector, body offset. */
+int fhe_log_to_ari_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_dev, int B,
+                            void *stream);
+int fhe_log_rescale_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_dev,
+                             uint32_t scale_bits, int B, void *stream);
+
 /* ---- host-pointer convenience forms (synchronous, like the *_st calls) -- */
 int fhe_pbs_lvl01(fhe_ctx *ctx, const uint32_t *in, const uint32_t *tv, int tv_stride,
                   uint32_t *out, int B);

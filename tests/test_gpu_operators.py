@@ -1,0 +1,93 @@
+"""GPU: level-generic bootstrap operators (cuHEDB::MSBGateBootstrapping /
+IdeGateBootstrapping for lvl1->lvl1, lvl2->lvl1, lvl2->lvl2; LOG_to_ARI;
+log_rescale), decrypted against plaintext truth.
+
+Semantics follow src/cuHEDB/gatebootstrap_gpu.cuh:31-170 (the GPU operators).
+Note: the reference's GPU LOG_to_ARI / log_rescale pass the test vector
+value u straight through (BlindRotate_gpu.cuh:92-122), while HEDB's CPU
+versions build it with mu_polygen, which negates its argument
+(src/HEDB/comparison/tfhepp_utils.h:105-111); the two therefore map logic
+true/false to opposite results.  This library implements the GPU operators.
+"""
+
+import numpy as np
+import pytest
+
+import cufhedb_amd as F
+import _oracle as O
+
+pytestmark = pytest.mark.gpu
+MU1 = 1 << 29
+
+
+@pytest.fixture(scope="module")
+def full_ctx(keys636):
+    k = keys636
+    ctx = F.Context(k.params)
+    ctx.load_bk(1, k.bk, F.BK_INT)
+    ctx.load_bk(2, F.keygen_bk_lvl02(k.sk, 636500), F.BK_INT)
+    ctx.load_ksk(10, k.ksk10)
+    ksk20 = F.keygen_ksk(k.sk, 20, 636720)
+    ctx.load_ksk(20, ksk20)
+    del ksk20
+    return k, ctx
+
+
+def enc2(k, m, seed):
+    return F.tlwe_encrypt_u64(seed, 0, k.sk.s2, np.asarray(m, np.uint64), 2.0 ** -52)
+
+
+def ph1(k, c):
+    return O.tdiff32(F.tlwe_phase_u32(c, k.sk.s1), 0)
+
+
+def ph2(k, c):
+    return O.tlwe_phase64(c, k.sk.s2).view(np.int64)
+
+
+def test_msb_from_lvl2(full_ctx, rng):
+    k, ctx = full_ctx
+    v = rng.integers(0, 16, 32)
+    c = enc2(k, v.astype(np.uint64) << np.uint64(60), 5001)
+    msb = (v >> 3) & 1
+    out = F.msb_gate_levels(ctx, 2, 1, c, F.LOGIC)
+    assert np.array_equal((ph1(k, out) > 0).astype(int), msb)
+    out = F.msb_gate_levels(ctx, 2, 2, c, F.LOGIC)
+    assert np.array_equal((ph2(k, out) > 0).astype(int), msb)
+    out = F.msb_gate_levels(ctx, 2, 2, c, F.ARITHMETIC)  # 1 -> 1/2, 0 -> 0
+    assert np.array_equal(np.round(ph2(k, out) / 2.0 ** 63).astype(np.int64) % 2, msb)
+
+
+def test_ide_from_lvl2(full_ctx, rng):
+    k, ctx = full_ctx
+    v = rng.integers(0, 16, 32)
+    c = enc2(k, v.astype(np.uint64) << np.uint64(59), 5002)
+    out = F.ide_gate_levels(ctx, 2, 1, c, 59)  # lvl1 scale 2^(59-32)
+    assert np.array_equal(np.round(ph1(k, out) / 2.0 ** 27).astype(np.int64) % 32, v)
+    v5 = rng.integers(0, 32, 32)
+    c5 = enc2(k, v5.astype(np.uint64) << np.uint64(58), 5003)
+    out = F.ide_gate_levels(ctx, 2, 2, c5, 58)  # gpolygen<lvl2>(5, 58)
+    assert np.array_equal(np.round(ph2(k, out) / 2.0 ** 58).astype(np.int64) % 64, v5)
+
+
+def test_msb_ide_lvl1_generic_matches_lvl1_entry_points(full_ctx, rng):
+    k, ctx = full_ctx
+    v = rng.integers(0, 16, 16)
+    c = k.enc_lvl1((v << 28).astype(np.uint32), seed=5004)
+    a = F.msb_gate_levels(ctx, 1, 1, c, F.LOGIC)
+    assert np.array_equal((ph1(k, a) > 0).astype(int), (v >> 3) & 1)
+    assert np.max(np.abs(ph1(k, a) - ph1(k, ctx.msb_gate(c, F.LOGIC)))) < 1 << 26
+
+
+def test_log_to_ari_and_log_rescale(full_ctx, rng):
+    k, ctx = full_ctx
+    bits = rng.integers(0, 2, 32)
+    c = k.enc_lvl1(np.where(bits == 1, MU1, (-MU1) % (1 << 32)).astype(np.uint32), seed=5005)
+    # LOG_to_ARI (GPU operator): tv = -2mu, body += mu
+    out = F.log_to_ari(ctx, c)
+    want = np.where(bits == 1, -MU1, 3 * MU1)
+    assert np.max(np.abs(ph1(k, out) - want)) < 1 << 26
+    # log_rescale(scale 29): tv = -2^28, body += 2^28 -> true 0, false 2^29
+    out = F.log_rescale(ctx, c, 29)
+    want = np.where(bits == 1, 0, 1 << 29)
+    assert np.max(np.abs(ph1(k, out) - want)) < 1 << 26
