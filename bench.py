@@ -109,6 +109,92 @@ def cpu_baseline(threads, seconds_budget=12.0):
     }
 
 
+def run_queries(F, ctx, sk, p, rows, reps):
+    """Encrypted query pipelines, each timed with HIP events on the context
+    stream over `reps` runs after one untimed run (which also sizes the
+    operator scratch); inputs resident on the device, results decrypted and
+    compared with the plaintext query."""
+    from cufhedb_amd import queries as Q
+    import ctypes
+
+    L_ = F.lib()
+    ctx.load_bk(2, F.keygen_bk_lvl02(sk, 20261205), F.BK_INT)
+    for kind, seed in ((10, 20261206), (20, 20261207), (21, 20261208)):
+        ksk = F.keygen_ksk(sk, kind, seed)
+        ctx.load_ksk(kind, ksk)
+        del ksk
+    rng = np.random.default_rng(13)
+    res = {}
+
+    def timed(fn):
+        fn()
+        ctx.sync()
+        F.check(L_.fhe_timer_start(ctx.h, None))
+        for _ in range(reps):
+            fn()
+        ms = ctypes.c_float()
+        F.check(L_.fhe_timer_stop(ctx.h, None, ctypes.byref(ms)))
+        return float(ms.value) / reps
+
+    # range filter x <= t, 20-bit values on the 64-bit torus, shared threshold
+    bits = 20
+    x = rng.integers(0, 1 << bits, rows)
+    t = int(rng.integers(0, 1 << bits))
+    sc = np.uint64(64 - bits - 1)
+    dx = F.DeviceBuffer.from_array(
+        F.tlwe_encrypt_u64(301, 0, sk.s2, x.astype(np.uint64) << sc, p.alpha2))
+    dt = F.DeviceBuffer.from_array(
+        F.tlwe_encrypt_u64(302, 0, sk.s2, np.array([t], np.uint64) << sc, p.alpha2))
+    out = F.DeviceBuffer(rows * (N + 1) * 4)
+    ms = timed(lambda: Q.range_filter(ctx, dx, dt, rows, bits, out_dev=out))
+    ph = F.tlwe_phase_u32(out.to_array(np.uint32, (rows, N + 1)), sk.s1).view(np.int32)
+    errors = int(np.sum((ph > 0) != (x <= t)))
+    res["range_filter"] = {
+        "workload": f"x <= t over {rows} rows of {bits}-bit lvl2 values, one shared encrypted "
+                    "threshold (HomMSB chain: 6 lvl2 + 3 lvl1 PBS per row)",
+        "rows": rows, "ms_per_query": ms, "errors": errors}
+    print(f"bench: range filter {rows} rows: {ms:.1f} ms/query, {errors} errors",
+          file=sys.stderr, flush=True)
+    for b in (dx, dt, out):
+        b.free()
+
+    # TPC-H Q1 filter: shipdate <= date, 4 (returnflag, linestatus) groups
+    date_bits = 9
+    rf = rng.integers(0, 2, rows)
+    ls = rng.integers(0, 2, rows)
+    date = rng.integers(0, 1 << date_bits, rows)
+    qty = rng.integers(0, 256, rows)
+    thr = int(rng.integers(0, 1 << date_bits))
+
+    def enc1(v, b, seed):
+        m = (np.asarray(v, np.uint64) << np.uint64(32 - b - 1)).astype(np.uint32)
+        return F.DeviceBuffer.from_array(F.tlwe_encrypt_u32(seed, 0, sk.s1, m, p.alpha1))
+
+    cols = [enc1(date, date_bits, 311), enc1(rf, 1, 312), enc1(ls, 1, 313)]
+    dthr, one = enc1([thr], date_bits, 314), enc1([1], 1, 315)
+    scratch = Q.Q1Scratch(rows)
+    ms = timed(lambda: Q.tpch_q1_filter(ctx, cols[0], cols[1], cols[2], dthr, one, rows,
+                                        scratch=scratch))
+    pred = Q.decrypt_predicates(scratch.out, rows, sk.s1)
+    ok = date <= thr
+    errors = 0
+    for g, (a, b) in enumerate(((0, 0), (0, 1), (1, 0), (1, 1))):
+        want = ok & (rf == a) & (ls == b)
+        errors += int(np.sum(pred[g] != want))
+        errors += int(int(np.sum(qty * pred[g])) != int(np.sum(qty[want])))
+    res["tpch_q1_filter"] = {
+        "workload": f"TPC-H Q1 filter on {rows} lvl1 rows: shipdate <= date (9 bit), "
+                    "4 (returnflag, linestatus) groups, log_rescale; SUM(qty)/COUNT after "
+                    "decryption",
+        "rows": rows, "ms_per_query": ms, "errors": errors}
+    print(f"bench: TPC-H Q1 filter {rows} rows: {ms:.1f} ms/query, {errors} errors",
+          file=sys.stderr, flush=True)
+    scratch.free()
+    for b in cols + [dthr, one]:
+        b.free()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +204,10 @@ def main():
     ap.add_argument("--preset", choices=["cfg636", "ref672"], default="cfg636")
     ap.add_argument("--variant", choices=["fp64"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-query", action="store_true",
+                    help="skip the encrypted-query pipelines (range filter, TPC-H Q1 filter)")
+    ap.add_argument("--query-rows", type=int, default=1 << 14)
+    ap.add_argument("--query-reps", type=int, default=2)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a plain copy "
                          "kernel of known size, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -217,6 +307,8 @@ def main():
         "decrypt_failures": failures,
         "device": F.device_name(),
     }
+    res["queries"] = (None if args.no_query else
+                      run_queries(F, ctx, sk, p, args.query_rows, args.query_reps))
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(threads)

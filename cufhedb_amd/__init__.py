@@ -116,6 +116,9 @@ _SIGS = {
     "fhe_ide_gate_dev": (_i, [_vp, _i, _i, _vp, _vp, _u32, _i, _vp]),
     "fhe_log_to_ari_lvl1_dev": (_i, [_vp, _vp, _vp, _i, _vp]),
     "fhe_log_rescale_lvl1_dev": (_i, [_vp, _vp, _vp, _u32, _i, _vp]),
+    "fhe_hom_msb_dev": (_i, [_vp, _i, _vp, _vp, _u32, _i, _i, _vp]),
+    "fhe_compare_dev": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _u32, _i, _i, _vp]),
+    "fhe_hom_not_dev": (_i, [_vp, _i, _vp, _vp, _i, _vp]),
     "fhe_pbs_lvl01": (_i, [_vp, _vp, _vp, _i, _vp, _i]),
     "fhe_iks": (_i, [_vp, _i, _vp, _vp, _i]),
     "fhe_malloc": (_i, [_vp, _sz]),
@@ -490,6 +493,48 @@ def log_rescale(ctx, a, scale_bits):
     a = np.ascontiguousarray(a, np.uint32)
     return ctx._run_dev(lib().fhe_log_rescale_lvl1_dev, [a], a.shape, np.uint32, scale_bits,
                         a.shape[0])
+
+
+# comparison operators (fhe_compare_dev), HomCompare_gpu.cuh:14-76
+CMP_GT, CMP_GE, CMP_LT, CMP_LE, CMP_EQ = 0, 1, 2, 3, 4
+
+
+def hom_msb(ctx, level, a, plain_bits, result_type=LOGIC):
+    """HomMSB (msb_gpu.cuh:51-57, 175-192): lvl1 or lvl2 TLWEs -> lvl1 TLWEs."""
+    a = np.ascontiguousarray(a, np.uint32 if level == 1 else np.uint64)
+    B = a.shape[0]
+    din = DeviceBuffer.from_array(a)
+    dout = DeviceBuffer(B * (LVL1_N + 1) * 4)
+    check(lib().fhe_hom_msb_dev(ctx.h, level, din.ptr, dout.ptr, plain_bits, result_type, B,
+                                None))
+    ctx.sync()
+    return dout.to_array(np.uint32, (B, LVL1_N + 1))
+
+
+def compare(ctx, level, op, a, b, plain_bits, result_type=LOGIC):
+    """a op b (CMP_*) for B pairs of lvl1/lvl2 TLWEs; b may be one TLWE
+    (broadcast to every a).  Returns lvl1 TLWEs."""
+    dt = np.uint32 if level == 1 else np.uint64
+    a = np.ascontiguousarray(a, dt)
+    b = np.ascontiguousarray(b, dt)
+    B = a.shape[0]
+    bcast = 1 if b.ndim == 1 else 0
+    da, db = DeviceBuffer.from_array(a), DeviceBuffer.from_array(b)
+    dout = DeviceBuffer(B * (LVL1_N + 1) * 4)
+    check(lib().fhe_compare_dev(ctx.h, level, op, da.ptr, db.ptr, bcast, dout.ptr, plain_bits,
+                                result_type, B, None))
+    ctx.sync()
+    return dout.to_array(np.uint32, (B, LVL1_N + 1))
+
+
+def hom_not(ctx, level, a):
+    """HomNOT<P>: every word negated (level 0, 1 or 2)."""
+    a = np.ascontiguousarray(a, np.uint64 if level == 2 else np.uint32)
+    din = DeviceBuffer.from_array(a)
+    dout = DeviceBuffer(a.nbytes)
+    check(lib().fhe_hom_not_dev(ctx.h, level, din.ptr, dout.ptr, a.shape[0], None))
+    ctx.sync()
+    return dout.to_array(a.dtype, a.shape)
 
 
 def device_name():

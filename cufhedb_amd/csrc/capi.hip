@@ -11,71 +11,10 @@
 #include <cstring>
 #include <vector>
 
-#include "common.hpp"
-#include "fft64.hpp"
+#include "ctx.hpp"
 #include "launch.hpp"
-#include "params.hpp"
 
 using namespace fhe;
-
-struct fhe_ctx {
-    fhe_params p{};
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    cd *bk1 = nullptr;  // lvl01 key, device FD layout
-    cd *bk2 = nullptr;  // lvl02 key, device FD layout
-    uint32_t *ksk[3] = {nullptr, nullptr, nullptr};  // kinds 10, 20, 21
-    cd *trgsw1 = nullptr;  // external-product TRGSW in device layout
-    cd *trgsw2 = nullptr;
-    int variant = FHE_PBS_FP64;
-    // grow-on-demand scratch for the bootstrap operators
-    uint32_t *tv1 = nullptr;     // one lvl1 test vector
-    uint64_t *tv2 = nullptr;     // one lvl2 test vector
-    uint32_t *tmp2 = nullptr;    // B x 2049 uint64 (as 2 x uint32 words)
-    size_t tmp2_cap = 0;
-    uint32_t *tmp1 = nullptr;    // B x 1025
-    size_t tmp1_cap = 0;
-    uint32_t *tmp0 = nullptr;    // B x (n0 + 1)
-    size_t tmp0_cap = 0;
-};
-
-namespace {
-
-int ksk_slot(int kind) { return kind == 10 ? 0 : kind == 20 ? 1 : kind == 21 ? 2 : -1; }
-
-int hip_fail(hipError_t e, const char *what)
-{
-    set_error("%s: %s", what, hipGetErrorString(e));
-    return e == hipErrorOutOfMemory ? FHE_ERR_ALLOC : FHE_ERR_DEVICE;
-}
-
-#define HIP_TRY(expr, what)                                 \
-    do {                                                    \
-        hipError_t _e = (expr);                             \
-        if (_e != hipSuccess) return hip_fail(_e, what);    \
-    } while (0)
-
-hipStream_t pick(fhe_ctx *ctx, void *stream)
-{
-    return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-}
-
-int grow(uint32_t *&buf, size_t &cap, size_t words)
-{
-    if (cap >= words) return FHE_OK;
-    if (buf) (void)hipFree(buf);
-    buf = nullptr;
-    cap = 0;
-    HIP_TRY(hipMalloc(&buf, words * sizeof(uint32_t)), "scratch allocation");
-    cap = words;
-    return FHE_OK;
-}
-
-int ks_target_n(const fhe_ctx *ctx, int kind) { return kind == 21 ? Lvl1::N : int(ctx->p.n0); }
-int ks_domain_words(int kind) { return kind == 10 ? Lvl1::N + 1 : Lvl2::N + 1; }
-
-}  // namespace
 
 extern "C" {
 
@@ -113,6 +52,8 @@ int fhe_ctx_destroy(fhe_ctx *ctx)
     void *bufs[] = {ctx->bk1, ctx->bk2, ctx->ksk[0], ctx->ksk[1], ctx->ksk[2], ctx->trgsw1,
                     ctx->trgsw2, ctx->tv1, ctx->tv2, ctx->tmp1, ctx->tmp0, ctx->tmp2};
     for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (uint32_t *b : ctx->chain)
         if (b) (void)hipFree(b);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -541,9 +482,12 @@ int fhe_log_to_ari_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_
                             void *stream)
 {
     FHE_CHECK_ARG(ctx && in_dev && out_dev && B >= 0, "fhe_log_to_ari_lvl1_dev: bad argument");
-    // gatebootstrap_gpu.cuh:155-162: tv = -2 mu, body += mu
-    return gate_pipeline(ctx, in_dev, nullptr, 1, 0, 0, 0u - (Lvl1::mu << 1), false, 0, Lvl1::mu,
-                         out_dev, B, pick(ctx, stream));
+    // HEDB LOG_to_ARI (tfhepp_utils.cpp:18-26): test vector mu_polygen(-2 mu)
+    // = +2 mu, body += mu, so true -> 3/8 and false -> -1/8 (1 and 0 at
+    // scale 1/2).  The reference's GPU twin (gatebootstrap_gpu.cuh:155-162)
+    // passes -2 mu as the constant itself, which inverts the result.
+    return gate_pipeline(ctx, in_dev, nullptr, 1, 0, 0, Lvl1::mu << 1, false, 0, Lvl1::mu, out_dev,
+                         B, pick(ctx, stream));
 }
 
 int fhe_log_rescale_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_dev,
@@ -551,9 +495,11 @@ int fhe_log_rescale_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out
 {
     FHE_CHECK_ARG(ctx && in_dev && out_dev && B >= 0, "fhe_log_rescale_lvl1_dev: bad argument");
     FHE_CHECK_ARG(scale_bits >= 1 && scale_bits <= 32, "fhe_log_rescale_lvl1_dev: scale_bits in [1, 32]");
-    // gatebootstrap_gpu.cuh:164-170: mu' = 2^(scale_bits-1), tv = -mu', body += mu'
+    // HEDB log_rescale (tfhepp_utils.cpp:28-35): mu' = 2^(scale_bits-1), test
+    // vector mu_polygen(-mu') = +mu', body += mu': true -> 2^scale_bits,
+    // false -> 0 (the GPU twin, gatebootstrap_gpu.cuh:164-170, inverts it)
     const uint32_t m = uint32_t(1ull << (scale_bits - 1));
-    return gate_pipeline(ctx, in_dev, nullptr, 1, 0, 0, 0u - m, false, 0, m, out_dev, B,
+    return gate_pipeline(ctx, in_dev, nullptr, 1, 0, 0, m, false, 0, m, out_dev, B,
                          pick(ctx, stream));
 }
 

@@ -1,0 +1,76 @@
+// ctx.hpp -- the device context behind the extern "C" boundary and the
+// helpers every entry-point file shares (error mapping, scratch growth).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+#include "fft64.hpp"
+#include "params.hpp"
+
+struct fhe_ctx {
+    fhe_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    fhe::cd *bk1 = nullptr;  // lvl01 key, device FD layout
+    fhe::cd *bk2 = nullptr;  // lvl02 key, device FD layout
+    uint32_t *ksk[3] = {nullptr, nullptr, nullptr};  // kinds 10, 20, 21
+    fhe::cd *trgsw1 = nullptr;  // external-product TRGSW in device layout
+    fhe::cd *trgsw2 = nullptr;
+    int variant = FHE_PBS_FP64;
+    // grow-on-demand scratch for the bootstrap operators
+    uint32_t *tv1 = nullptr;     // one lvl1 test vector
+    uint64_t *tv2 = nullptr;     // one lvl2 test vector
+    uint32_t *tmp2 = nullptr;    // B x 2049 uint64 (as 2 x uint32 words)
+    size_t tmp2_cap = 0;
+    uint32_t *tmp1 = nullptr;    // B x 1025
+    size_t tmp1_cap = 0;
+    uint32_t *tmp0 = nullptr;    // B x (n0 + 1)
+    size_t tmp0_cap = 0;
+    // scratch of the comparison chains (HomMSB / compare), kept apart from
+    // the single-operator scratch above because a chain step feeds one
+    // operator's output into the next
+    static constexpr int kChainBufs = 7;
+    uint32_t *chain[kChainBufs] = {};
+    size_t chain_cap[kChainBufs] = {};
+};
+
+namespace fhe {
+
+inline int ksk_slot(int kind) { return kind == 10 ? 0 : kind == 20 ? 1 : kind == 21 ? 2 : -1; }
+
+inline int hip_fail(hipError_t e, const char *what)
+{
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? FHE_ERR_ALLOC : FHE_ERR_DEVICE;
+}
+
+#define HIP_TRY(expr, what)                                 \
+    do {                                                    \
+        hipError_t _e = (expr);                             \
+        if (_e != hipSuccess) return hip_fail(_e, what);    \
+    } while (0)
+
+inline hipStream_t pick(fhe_ctx *ctx, void *stream)
+{
+    return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+}
+
+inline int grow(uint32_t *&buf, size_t &cap, size_t words)
+{
+    if (cap >= words) return FHE_OK;
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    cap = 0;
+    HIP_TRY(hipMalloc(&buf, words * sizeof(uint32_t)), "scratch allocation");
+    cap = words;
+    return FHE_OK;
+}
+
+inline int ks_target_n(const fhe_ctx *ctx, int kind) { return kind == 21 ? Lvl1::N : int(ctx->p.n0); }
+inline int ks_domain_words(int kind) { return kind == 10 ? Lvl1::N + 1 : Lvl2::N + 1; }
+
+}  // namespace fhe
+

@@ -13,10 +13,16 @@
 //   freq domain : lane s, register q holds Z[(s>>3) + 8 (s&7) + 64 q]
 // 512 = 8 x 8 x 8.  Pass 1 is a radix-8 DFT over the 8 registers of a lane;
 // the twist w^n is folded into the constants c_j = w^{64 j} before it and
-// the per-lane twiddle w^{L(4k+1)} after it.  Two transposes through a
-// 8 x 65 double2 LDS tile (row pad of one element keeps every aligned group
-// of 8 lanes on 32 distinct banks for ds_read/write_b128) regroup the data
-// for pass 2 (radix 8, twiddle W64^{L0 p}) and pass 3 (radix 8).  The
+// the per-lane twiddle w^{L(4k+1)} after it.  Two transposes through an
+// 8-row double2 LDS tile regroup the data for pass 2 (radix 8, twiddle
+// W64^{L0 p}) and pass 3 (radix 8).  Row strides are chosen for the LDS of
+// gfx950, which serves a ds_read/write_b128 16 lanes (256 bytes, all 64
+// banks) per cycle: the strided side of transpose 1 has lanes (L0, k1)
+// reading row k1, column L0 + 8 l1, so the row stride is 72 double2
+// (= 32 banks mod 64) and the two rows one cycle touches fill opposite bank
+// halves; the strided side of transpose 2 has lanes (p, k1) reading row p,
+// column l0 + 8 k1, so that stride is 65 (4 banks mod 64 per row, the
+// k1 parity adding 32).  Row-contiguous sides are conflict-free.  The
 // inverse runs the same steps mirrored with conjugate twiddles, so it takes
 // the permuted frequency layout and returns natural time order.
 // Twiddle powers are generated in registers from three per-lane bases, so
@@ -162,8 +168,9 @@ __device__ __forceinline__ void store_twiddles(cd *t, int lane, const Twiddles10
     t[128 + lane] = tw.w64;
 }
 
-constexpr int kFftPad = 65;                 // LDS row stride in double2
-constexpr int kFftLdsElems = 8 * kFftPad;   // 520 double2 = 8320 bytes
+constexpr int kFftPad = 65;                  // row stride of transpose 2 (double2)
+constexpr int kFftPadA = 72;                 // row stride of transpose 1 (double2)
+constexpr int kFftLdsElems = 8 * kFftPadA;   // 576 double2 = 9216 bytes
 
 // Forward transform: v[j] = z[L + 64 j] on entry, Z in permuted layout on exit.
 // TWIST = true: twisted negacyclic map above (w = exp(i pi/1024)).
@@ -188,12 +195,12 @@ __device__ __forceinline__ void fft512_fwd(cd v[8], cd *lds, int lane, const Twi
     }
     // transpose 1: T[k1][L] -> lane (L0 + 8 k1) register L1
 #pragma unroll
-    for (int k = 0; k < 8; k++) lds[k * kFftPad + lane] = v[k];
+    for (int k = 0; k < 8; k++) lds[k * kFftPadA + lane] = v[k];
     __syncthreads();
     {
         const int L0 = lane & 7, k1 = lane >> 3;
 #pragma unroll
-        for (int l1 = 0; l1 < 8; l1++) v[l1] = lds[k1 * kFftPad + L0 + 8 * l1];
+        for (int l1 = 0; l1 < 8; l1++) v[l1] = lds[k1 * kFftPadA + L0 + 8 * l1];
     }
     __syncthreads();
     dft8<1>(v);
@@ -246,11 +253,11 @@ __device__ __forceinline__ void fft512_inv(cd v[8], cd *lds, int lane, const Twi
     {
         const int L0 = lane & 7, k1 = lane >> 3;
 #pragma unroll
-        for (int l1 = 0; l1 < 8; l1++) lds[k1 * kFftPad + L0 + 8 * l1] = v[l1];
+        for (int l1 = 0; l1 < 8; l1++) lds[k1 * kFftPadA + L0 + 8 * l1] = v[l1];
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 8; k++) v[k] = lds[k * kFftPad + lane];
+    for (int k = 0; k < 8; k++) v[k] = lds[k * kFftPadA + lane];
     __syncthreads();
     {
         cd t = tw.wl;

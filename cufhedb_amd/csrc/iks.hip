@@ -100,6 +100,29 @@ __global__ void k_tlwe_copy_add_body(const T *__restrict__ in, T *__restrict__ o
     out[gid] = v;
 }
 
+// out = sa*a + sb*b (+ add on the body word); W words per TLWE.  b == nullptr
+// drops the second term; b_bcast makes every record use b's first TLWE.
+template <class T>
+__global__ void k_tlwe_lincomb(const T *__restrict__ a, const T *__restrict__ b, int sa, int sb,
+                               T add, T *__restrict__ out, int W, int b_bcast, int B)
+{
+    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= size_t(B) * W) return;
+    const int e = int(gid % W);
+    T v = T(sa) * a[gid];
+    if (b) v += T(sb) * b[b_bcast ? size_t(e) : gid];
+    if (e == W - 1) v += add;
+    out[gid] = v;
+}
+
+// out = in << shift on every word (HomMSB's digit shift, msb_gpu.cuh:15-18)
+template <class T>
+__global__ void k_tlwe_shl(const T *__restrict__ in, T *__restrict__ out, int shift, size_t n)
+{
+    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid < n) out[gid] = in[gid] << shift;
+}
+
 template <class T>
 __global__ void k_tlwe_add_body(T *__restrict__ tlwe, int W, T add, int B)
 {
@@ -137,6 +160,35 @@ template hipError_t launch_tlwe_copy_add_body<uint32_t>(const uint32_t *, uint32
                                                         int, hipStream_t);
 template hipError_t launch_tlwe_copy_add_body<uint64_t>(const uint64_t *, uint64_t *, int, uint64_t,
                                                         int, hipStream_t);
+
+template <class T>
+hipError_t launch_tlwe_lincomb(const T *a, const T *b, int sa, int sb, T add, T *out, int W,
+                               bool b_bcast, int B, hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    const size_t n = size_t(B) * W;
+    hipLaunchKernelGGL(k_tlwe_lincomb<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, a, b,
+                       sa, sb, add, out, W, int(b_bcast), B);
+    return hipGetLastError();
+}
+template hipError_t launch_tlwe_lincomb<uint32_t>(const uint32_t *, const uint32_t *, int, int,
+                                                  uint32_t, uint32_t *, int, bool, int, hipStream_t);
+template hipError_t launch_tlwe_lincomb<uint64_t>(const uint64_t *, const uint64_t *, int, int,
+                                                  uint64_t, uint64_t *, int, bool, int, hipStream_t);
+
+template <class T>
+hipError_t launch_tlwe_shl(const T *in, T *out, int W, int shift, int B, hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    const size_t n = size_t(B) * W;
+    hipLaunchKernelGGL(k_tlwe_shl<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, in, out,
+                       shift, n);
+    return hipGetLastError();
+}
+template hipError_t launch_tlwe_shl<uint32_t>(const uint32_t *, uint32_t *, int, int, int,
+                                              hipStream_t);
+template hipError_t launch_tlwe_shl<uint64_t>(const uint64_t *, uint64_t *, int, int, int,
+                                              hipStream_t);
 
 template <class T>
 hipError_t launch_tlwe_add_body(T *tlwe, int W, T add, int B, hipStream_t s)

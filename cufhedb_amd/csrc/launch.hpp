@@ -59,6 +59,12 @@ template <class T>
 hipError_t launch_tlwe_copy_add_body(const T *in, T *out, int W, T add, int B, hipStream_t s);
 template <class T>
 hipError_t launch_tlwe_add_body(T *tlwe, int W, T add, int B, hipStream_t s);
+// out = sa*a + sb*b (+ add on the body); b may be null; b_bcast: b is one TLWE
+template <class T>
+hipError_t launch_tlwe_lincomb(const T *a, const T *b, int sa, int sb, T add, T *out, int W,
+                               bool b_bcast, int B, hipStream_t s);
+template <class T>
+hipError_t launch_tlwe_shl(const T *in, T *out, int W, int shift, int B, hipStream_t s);
 template <class T>
 hipError_t launch_tv_const(T *tv, T u, int N, hipStream_t s);
 template <class T>

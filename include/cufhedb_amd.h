@@ -188,6 +188,8 @@ enum fhe_gate {
     FHE_GATE_NOR = 3,  /* TFHEpp HomNOR, gate.cpp:73-78                    */
     FHE_GATE_XNOR = 4  /* TFHEpp HomXNOR, gate.cpp:86-91                   */
 };
+/* result_type of the operators (HEDB/utils/types.h:4-5) */
+enum fhe_result_type { FHE_ARITHMETIC = 0, FHE_LOGIC = 1 };
 /* Two-input gate on lvl1 TLWEs: linear combination, IKS lvl10, PBS.
  * result_type: 1 = LOGIC (+-mu), 0 = ARITHMETIC (0 / 2mu), as in
  * HEDB/utils/types.h:4-11; NAND/NOR/XNOR are always LOGIC. */
@@ -210,11 +212,45 @@ int fhe_msb_gate_dev(fhe_ctx *ctx, int in_level, int out_level, const void *in_d
 int fhe_ide_gate_dev(fhe_ctx *ctx, int in_level, int out_level, const void *in_dev, void *out_dev,
                      uint32_t scale_bits, int B, void *stream);
 /* cuHEDB::LOG_to_ARI (gatebootstrap_gpu.cuh:155-162) and cuHEDB::log_rescale
- * (:164-170): IKS lvl10, PBS with a constant test vector, body offset. */
+ * (:164-170): IKS lvl10, PBS with a constant test vector, body offset.  The
+ * test vector follows HEDB's CPU definitions (tfhepp_utils.cpp:18-35,
+ * mu_polygen(-mu) = +mu): LOG_to_ARI maps true/false to 3/8 and -1/8 (1 and
+ * 0 at scale 1/2), log_rescale to 2^scale_bits and 0.  The GPU twins pass
+ * -mu unnegated and so return the inverted bit. */
 int fhe_log_to_ari_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_dev, int B,
                             void *stream);
 int fhe_log_rescale_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_dev,
                              uint32_t scale_bits, int B, void *stream);
+
+/* ---- comparison layer (cuHEDB msb_gpu.cuh, HomCompare_gpu.cuh) ---------- */
+/* HomMSB (msb_gpu.cuh:51-57 lvl1, :175-192 lvl2; CPU twin
+ * HEDB/comparison/extract_msb.cpp:241-259): encrypted top bit of a
+ * plaintext of `plain_bits` bits held in the top bits of the torus, by 5-bit
+ * peeling steps (shift, MSB gate, identity bootstrapping, subtract) down to
+ * one MSB gate.  in_level 1: plain_bits <= 10; in_level 2: plain_bits <= 33,
+ * through IKS lvl21 once at most 9 bits remain.  Output: lvl1 TLWEs.
+ * Needs BK lvl01 + KSK 10, and for level 2 also BK lvl02 + KSK 20 and 21. */
+int fhe_hom_msb_dev(fhe_ctx *ctx, int in_level, const void *in_dev, uint32_t *out_dev,
+                    uint32_t plain_bits, int result_type, int B, void *stream);
+enum fhe_cmp {
+    FHE_CMP_GT = 0, /* cuHEDB::greater_than,       HomCompare_gpu.cuh:14-24 */
+    FHE_CMP_GE = 1, /* cuHEDB::greater_than_equal, HomCompare_gpu.cuh:27-37 */
+    FHE_CMP_LT = 2, /* cuHEDB::less_than,          HomCompare_gpu.cuh:40-49 */
+    FHE_CMP_LE = 3, /* cuHEDB::less_than_equal,    HomCompare_gpu.cuh:52-61 */
+    FHE_CMP_EQ = 4  /* cuHEDB::equal,              HomCompare_gpu.cuh:64-72 */
+};
+/* a op b for B pairs of TLWEs of `level` holding plain_bits-bit values at
+ * scale 2^(w - plain_bits - 1) (one padding bit, as in
+ * test/HE3_Query/HomComp.cu:75-80).  b_broadcast != 0 compares every a with
+ * the single TLWE at b_dev (a shared or public threshold).  Output: lvl1
+ * TLWEs, LOGIC or ARITHMETIC. */
+int fhe_compare_dev(fhe_ctx *ctx, int level, int op, const void *a_dev, const void *b_dev,
+                    int b_broadcast, uint32_t *out_dev, uint32_t plain_bits, int result_type,
+                    int B, void *stream);
+/* HomNOT<P> (gatebootstrap_gpu.cuh:117-121; TFHEpp gate.cpp:30-34): negate
+ * every word of B TLWEs of level 0, 1 or 2 (out may equal in). */
+int fhe_hom_not_dev(fhe_ctx *ctx, int level, const void *in_dev, void *out_dev, int B,
+                    void *stream);
 
 /* ---- host-pointer convenience forms (synchronous, like the *_st calls) -- */
 int fhe_pbs_lvl01(fhe_ctx *ctx, const uint32_t *in, const uint32_t *tv, int tv_stride,

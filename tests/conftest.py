@@ -96,6 +96,31 @@ def ctx636():
     return ctx_for(F.PRESET_CFG636)
 
 
+_FULL = {}
+
+
+def full_ctx_for(which):
+    """Context holding every key the operator layer uses: BK lvl01 and lvl02,
+    KSK 10, 20 and 21 (host copies of the KSKs are dropped after upload)."""
+    if which not in _FULL:
+        k = keys_for(which)
+        ctx = F.Context(k.params)
+        ctx.load_bk(1, k.bk, F.BK_INT)
+        ctx.load_bk(2, F.keygen_bk_lvl02(k.sk, k.seed + 3), F.BK_INT)
+        ctx.load_ksk(10, k.ksk10)
+        for kind, off in ((20, 4), (21, 5)):
+            ksk = F.keygen_ksk(k.sk, kind, k.seed + off)
+            ctx.load_ksk(kind, ksk)
+            del ksk
+        _FULL[which] = ctx
+    return keys_for(which), _FULL[which]
+
+
+@pytest.fixture(scope="session")
+def full636():
+    return full_ctx_for(F.PRESET_CFG636)
+
+
 @pytest.fixture
 def rng():
     return np.random.default_rng(20261202)

@@ -7,7 +7,9 @@ Note: the reference's GPU LOG_to_ARI / log_rescale pass the test vector
 value u straight through (BlindRotate_gpu.cuh:92-122), while HEDB's CPU
 versions build it with mu_polygen, which negates its argument
 (src/HEDB/comparison/tfhepp_utils.h:105-111); the two therefore map logic
-true/false to opposite results.  This library implements the GPU operators.
+true/false to opposite results.  This library follows the CPU definitions,
+under which true maps to arithmetic 1 (greater_than_equal in ARITHMETIC mode
+depends on it).
 """
 
 import numpy as np
@@ -21,16 +23,8 @@ MU1 = 1 << 29
 
 
 @pytest.fixture(scope="module")
-def full_ctx(keys636):
-    k = keys636
-    ctx = F.Context(k.params)
-    ctx.load_bk(1, k.bk, F.BK_INT)
-    ctx.load_bk(2, F.keygen_bk_lvl02(k.sk, 636500), F.BK_INT)
-    ctx.load_ksk(10, k.ksk10)
-    ksk20 = F.keygen_ksk(k.sk, 20, 636720)
-    ctx.load_ksk(20, ksk20)
-    del ksk20
-    return k, ctx
+def full_ctx(full636):
+    return full636
 
 
 def enc2(k, m, seed):
@@ -83,11 +77,12 @@ def test_log_to_ari_and_log_rescale(full_ctx, rng):
     k, ctx = full_ctx
     bits = rng.integers(0, 2, 32)
     c = k.enc_lvl1(np.where(bits == 1, MU1, (-MU1) % (1 << 32)).astype(np.uint32), seed=5005)
-    # LOG_to_ARI (GPU operator): tv = -2mu, body += mu
+    # LOG_to_ARI (tfhepp_utils.cpp:18-26): tv = +2mu, body += mu
     out = F.log_to_ari(ctx, c)
-    want = np.where(bits == 1, -MU1, 3 * MU1)
+    want = np.where(bits == 1, 3 * MU1, -MU1)
     assert np.max(np.abs(ph1(k, out) - want)) < 1 << 26
-    # log_rescale(scale 29): tv = -2^28, body += 2^28 -> true 0, false 2^29
+    assert np.array_equal(np.round(ph1(k, out) / 2.0 ** 31).astype(np.int64) % 2, bits)
+    # log_rescale(scale 29): tv = +2^28, body += 2^28 -> true 2^29, false 0
     out = F.log_rescale(ctx, c, 29)
-    want = np.where(bits == 1, 0, 1 << 29)
+    want = np.where(bits == 1, 1 << 29, 0)
     assert np.max(np.abs(ph1(k, out) - want)) < 1 << 26
