@@ -153,21 +153,6 @@ __device__ __forceinline__ Twiddles1024 make_twiddles512_plain(int lane)
     return t;
 }
 
-// Twiddle bases kept in LDS (3 x 64 complex per wave) instead of registers.
-struct TwLds {
-    const cd *t;  // [3][64]: wl, w4l, w64
-    __device__ __forceinline__ Twiddles1024 get(int lane) const
-    {
-        return Twiddles1024{t[lane], t[64 + lane], t[128 + lane]};
-    }
-};
-__device__ __forceinline__ void store_twiddles(cd *t, int lane, const Twiddles1024 &tw)
-{
-    t[lane] = tw.wl;
-    t[64 + lane] = tw.w4l;
-    t[128 + lane] = tw.w64;
-}
-
 constexpr int kFftPad = 65;                  // row stride of transpose 2 (double2)
 constexpr int kFftPadA = 72;                 // row stride of transpose 1 (double2)
 constexpr int kFftLdsElems = 8 * kFftPadA;   // 576 double2 = 9216 bytes

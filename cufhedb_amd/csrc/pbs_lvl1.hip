@@ -73,14 +73,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 {
     __shared__ uint32_t acc[2 * N1];
     __shared__ cd tile[kFftLdsElems];
-    __shared__ cd twl[3 * 64];
     const int lane = threadIdx.x;
     const int ct = blockIdx.x;
     if (ct >= B) return;
     const uint32_t *c0 = in + size_t(ct) * (n0 + 1);
     const uint32_t *tvp = tv + size_t(ct) * tv_stride;
-    store_twiddles(twl, lane, make_twiddles1024(lane));
-    const TwLds twp{twl};
+    const Twiddles1024 tw = make_twiddles1024(lane);
 
     // acc = (0, X^bbar * tv), bbar = 2N - (b >> 21)   (no rounding)
     const uint32_t bbar = 2u * N1 - (c0[n0] >> (32 - 1 - Lvl1::nbit));
@@ -103,42 +101,36 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 #pragma unroll
         for (int q = 0; q < 8; q++) fd0[q] = fd1[q] = cd{0.0, 0.0};
 #pragma unroll 1
-        for (int c = 0; c < 2; c++) {
-            // (X^abar - 1) acc_c, offset and flipped for digit extraction,
-            // read once per polynomial from LDS
+        for (int r = 0; r < Lvl1::rows; r++) {
+            const int c = r / Lvl1::l, d = r - c * Lvl1::l;
             const uint32_t *poly = acc + c * N1;
-            uint32_t t[16];
+            // issue this row's key loads before the FFT so their latency
+            // overlaps the transform
+            cd b0[8], b1[8];
+            load_row(bki + r * 2 * kPolyCd, lane, b0, b1);
+            // digits of (X^abar - 1) acc_c, recomputed per row from LDS
+            // (cheaper in registers than holding the rotated polynomial)
+            cd v[8];
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                t[2 * j] = prep1(rot_minus_one(poly, abar, lane + 64 * j));
-                t[2 * j + 1] = prep1(rot_minus_one(poly, abar, lane + 64 * j + 512));
-            }
-#pragma unroll 1
-            for (int d = 0; d < Lvl1::l; d++) {
-                // issue this row's key loads before the FFT so their latency
-                // overlaps the transform
-                cd b0[8], b1[8];
-                load_row(bki + (c * Lvl1::l + d) * 2 * kPolyCd, lane, b0, b1);
-                cd v[8];
+            for (int j = 0; j < 8; j++)
+                v[j] = cd{digit1(rot_minus_one(poly, abar, lane + 64 * j), d),
+                          digit1(rot_minus_one(poly, abar, lane + 64 * j + 512), d)};
+            fft512_fwd(v, tile, lane, tw);
 #pragma unroll
-                for (int j = 0; j < 8; j++) v[j] = cd{digit1p(t[2 * j], d), digit1p(t[2 * j + 1], d)};
-                fft512_fwd(v, tile, lane, twp.get(lane));
-#pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    cmac(fd0[q], v[q], b0[q]);
-                    cmac(fd1[q], v[q], b1[q]);
-                }
+            for (int q = 0; q < 8; q++) {
+                cmac(fd0[q], v[q], b0[q]);
+                cmac(fd1[q], v[q], b1[q]);
             }
         }
         // every lane's rotation reads are complete (the FFTs synchronised)
-        fft512_inv(fd0, tile, lane, twp.get(lane));
+        fft512_inv(fd0, tile, lane, tw);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int n = lane + 64 * j;
             acc[n] += trunc_torus32(fd0[j].x);
             acc[n + 512] += trunc_torus32(fd0[j].y);
         }
-        fft512_inv(fd1, tile, lane, twp.get(lane));
+        fft512_inv(fd1, tile, lane, tw);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int n = lane + 64 * j;
