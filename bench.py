@@ -202,7 +202,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--preset", choices=["cfg636", "ref672"], default="cfg636")
-    ap.add_argument("--variant", choices=["fp64"], default="fp64")
+    ap.add_argument("--variant", choices=["fp64", "mfma_split", "both"], default="both",
+                    help="lvl1 PBS kernel variant(s) to time; 'both' reports each and takes the "
+                         "faster for the headline value")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-query", action="store_true",
                     help="skip the encrypted-query pipelines (range filter, TPC-H Q1 filter)")
@@ -227,7 +229,6 @@ def main():
     bk = F.keygen_bk_lvl01(sk, 20261203)
     ctx = F.Context(p, 0)
     ctx.load_bk(1, bk, F.BK_INT)
-    ctx.set_pbs_variant({"fp64": F.PBS_FP64}[args.variant])
 
     B = args.batch
     rng = np.random.default_rng(11)
@@ -251,25 +252,45 @@ def main():
         src.free()
         dst.free()
 
-    for _ in range(args.warmup):
-        step()
-    ctx.sync()
-    print(f"bench: warmup done ({args.warmup} x {B} PBS)", file=sys.stderr, flush=True)
-    F.check(L_.fhe_timer_start(ctx.h, None))
-    for _ in range(args.steps):
-        step()
-    ms = __import__("ctypes").c_float()
-    F.check(L_.fhe_timer_stop(ctx.h, None, __import__("ctypes").byref(ms)))
-    total_ms = float(ms.value)
-    ms_per_step = total_ms / args.steps
-    value = B * args.steps / (total_ms / 1e3)
-
-    out = d_out.to_array(np.uint32, (B, N + 1))
-    ph = F.tlwe_phase_u32(out, sk.s1)
-    dec = np.round(ph.view(np.int32).astype(np.int64) / (1 << 28)).astype(np.int64) % 16
-    failures = int(np.sum(dec != lut[m]))
-    print(f"bench: timed {args.steps} steps, {ms_per_step:.3f} ms/step, "
-          f"{failures} decryption failures", file=sys.stderr, flush=True)
+    codes = {"fp64": F.PBS_FP64, "mfma_split": F.PBS_MFMA_SPLIT}
+    names = ["fp64", "mfma_split"] if args.variant == "both" else [args.variant]
+    variants = {}
+    for name in names:
+        ctx.set_pbs_variant(codes[name])
+        for _ in range(args.warmup):
+            step()
+        ctx.sync()
+        print(f"bench: {name}: warmup done ({args.warmup} x {B} PBS)", file=sys.stderr,
+              flush=True)
+        F.check(L_.fhe_timer_start(ctx.h, None))
+        for _ in range(args.steps):
+            step()
+        ms = __import__("ctypes").c_float()
+        F.check(L_.fhe_timer_stop(ctx.h, None, __import__("ctypes").byref(ms)))
+        total_ms = float(ms.value)
+        out = d_out.to_array(np.uint32, (B, N + 1))
+        ph = F.tlwe_phase_u32(out, sk.s1).view(np.int32).astype(np.int64)
+        dec = np.round(ph / (1 << 28)).astype(np.int64) % 16
+        err = ph - lut[m] * (1 << 28)
+        variants[name] = {
+            "ms_per_step": total_ms / args.steps,
+            "pbs_per_s": B * args.steps / (total_ms / 1e3),
+            "decrypt_failures": int(np.sum(dec != lut[m])),
+            "phase_error_std_log2": float(np.log2(np.std(err))),
+        }
+        print(f"bench: {name}: timed {args.steps} steps, {total_ms / args.steps:.3f} ms/step, "
+              f"{variants[name]['decrypt_failures']} decryption failures", file=sys.stderr,
+              flush=True)
+    # the headline is the fastest variant that decrypts every output
+    exact = [k for k in variants if variants[k]["decrypt_failures"] == 0] or list(variants)
+    best = max(exact, key=lambda k: variants[k]["pbs_per_s"])
+    ms_per_step = variants[best]["ms_per_step"]
+    value = variants[best]["pbs_per_s"]
+    failures = variants[best]["decrypt_failures"]
+    if len(variants) == 2:
+        variants["mfma_split_over_fp64"] = (variants["mfma_split"]["pbs_per_s"] /
+                                            variants["fp64"]["pbs_per_s"])
+    ctx.set_pbs_variant(codes[best])
 
     fl = flops_per_pbs(p.n0)
     achieved = fl * B / (ms_per_step / 1e3) / 1e12
@@ -279,8 +300,11 @@ def main():
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP64_TFLOPS,
-        "traffic": load_traffic(args.variant, args.preset, B),
-        "kernel": "k_blind_rotate1 (FP64 VALU; MI355X FP64 vector == FP64 matrix peak)",
+        "traffic": load_traffic(best, args.preset, B),
+        "kernel": ("k_blind_rotate1 (FP64 VALU; MI355X FP64 vector == FP64 matrix peak)"
+                   if best == "fp64" else
+                   "k_blind_rotate1_mfma (fp16 MFMA forward transforms; bound by its FP64/FP32 "
+                   "VALU work, priced at the FP64 peak)"),
         "flops_per_pbs": fl,
     }
     res = {
@@ -294,16 +318,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if best == "fp64" else "f16-split/f32/f64",
         "data": "synthetic (seeded keys and Z_8 plaintexts, random 8-entry LUT)",
         "config": {
             "workload": f"lvl1 PBS (BlindRotate+SampleExtract), batch {B}, n0={p.n0}, "
-                        f"N=1024 k=1 l=3 Bgbit=6, {args.variant}",
+                        f"N=1024 k=1 l=3 Bgbit=6, {best}",
             "batch": B,
             "n0": int(p.n0),
-            "variant": args.variant,
+            "variant": best,
         },
         "roofline": roofline,
+        "variants": variants,
         "decrypt_failures": failures,
         "device": F.device_name(),
     }

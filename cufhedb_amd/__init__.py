@@ -39,6 +39,7 @@ PRESET_CFG636 = 1
 BK_INT = 0
 BK_FFT = 1
 PBS_FP64 = 0
+PBS_MFMA_SPLIT = 1
 
 GATE_NAND, GATE_AND, GATE_OR, GATE_NOR, GATE_XNOR = range(5)
 LOGIC = 1
@@ -117,6 +118,7 @@ _SIGS = {
     "fhe_log_to_ari_lvl1_dev": (_i, [_vp, _vp, _vp, _i, _vp]),
     "fhe_log_rescale_lvl1_dev": (_i, [_vp, _vp, _vp, _u32, _i, _vp]),
     "fhe_hom_msb_dev": (_i, [_vp, _i, _vp, _vp, _u32, _i, _i, _vp]),
+    "fhe_fwd_digits_lvl1_mfma_dev": (_i, [_vp, _vp, _vp, _i, _vp]),
     "fhe_compare_dev": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _u32, _i, _i, _vp]),
     "fhe_hom_not_dev": (_i, [_vp, _i, _vp, _vp, _i, _vp]),
     "fhe_pbs_lvl01": (_i, [_vp, _vp, _vp, _i, _vp, _i]),
@@ -390,6 +392,14 @@ class Context:
         return self._run_dev(lib().fhe_external_product_lvl1_dev,
                              [trlwe, np.ascontiguousarray(trgswfft, np.float64)],
                              (B, 2, LVL1_N), np.uint32, B)
+
+    def fwd_digits_lvl1_mfma(self, poly):
+        """Split-MFMA forward transform of small-integer polynomials (B x 1024
+        int32, |x| <= 2048) -> B x 1024 FD in TFHEpp order."""
+        poly = np.ascontiguousarray(poly, np.int32)
+        B = poly.shape[0]
+        return self._run_dev(lib().fhe_fwd_digits_lvl1_mfma_dev, [poly], (B, LVL1_N), np.float64,
+                             B)
 
     def pbs_lvl02(self, tlwe0, tv):
         tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)

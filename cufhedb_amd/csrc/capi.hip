@@ -16,6 +16,45 @@
 
 using namespace fhe;
 
+namespace {
+
+// The split-MFMA variant's key (FP32, its own frequency order) and tables,
+// derived from the FP64 device key on first use after each lvl1 upload.
+int ensure_mfma(fhe_ctx *ctx, hipStream_t s)
+{
+    if (!ctx->mfma_tab) {
+        HIP_TRY(hipMalloc(&ctx->mfma_tab, mfma_tables_bytes()), "MFMA table allocation");
+        HIP_TRY(launch_mfma_tables(ctx->mfma_tab, s), "MFMA tables");
+    }
+    if (!ctx->bk1m_valid) {
+        if (!ctx->bk1m)
+            HIP_TRY(hipMalloc(&ctx->bk1m, bk1_mfma_floats(int(ctx->p.n0)) * sizeof(float)),
+                    "MFMA key allocation");
+        HIP_TRY(launch_bk1_to_mfma(ctx->bk1, ctx->bk1m, int(ctx->p.n0), s), "MFMA key layout");
+        ctx->bk1m_valid = true;
+    }
+    return FHE_OK;
+}
+
+// lvl1 blind rotation (+ sample extraction) with the context's variant
+int launch_pbs1(fhe_ctx *ctx, const uint32_t *in, const uint32_t *tv, int tv_stride,
+                uint32_t *out, bool se, int B, hipStream_t s)
+{
+    const int n0 = int(ctx->p.n0);
+    if (ctx->variant == FHE_PBS_MFMA_SPLIT) {
+        int rc = ensure_mfma(ctx, s);
+        if (rc) return rc;
+        HIP_TRY(launch_blind_rotate1_mfma(in, n0, ctx->bk1m, ctx->mfma_tab, tv, tv_stride, out, se,
+                                          B, s),
+                "pbs lvl01 (split MFMA)");
+        return FHE_OK;
+    }
+    HIP_TRY(launch_blind_rotate1(in, n0, ctx->bk1, tv, tv_stride, out, se, B, s), "pbs lvl01");
+    return FHE_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int fhe_ctx_create(const fhe_params *p, int device, fhe_ctx **out)
@@ -49,8 +88,9 @@ int fhe_ctx_destroy(fhe_ctx *ctx)
     if (!ctx) return FHE_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    void *bufs[] = {ctx->bk1, ctx->bk2, ctx->ksk[0], ctx->ksk[1], ctx->ksk[2], ctx->trgsw1,
-                    ctx->trgsw2, ctx->tv1, ctx->tv2, ctx->tmp1, ctx->tmp0, ctx->tmp2};
+    void *bufs[] = {ctx->bk1,  ctx->bk2,    ctx->ksk[0], ctx->ksk[1], ctx->ksk[2],
+                    ctx->trgsw1, ctx->trgsw2, ctx->tv1,  ctx->tv2,    ctx->tmp1,
+                    ctx->tmp0, ctx->tmp2,   ctx->bk1m, ctx->mfma_tab};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (uint32_t *b : ctx->chain)
@@ -74,7 +114,8 @@ void *fhe_ctx_stream(fhe_ctx *ctx) { return ctx ? static_cast<void *>(ctx->strea
 int fhe_set_pbs_variant(fhe_ctx *ctx, int variant)
 {
     FHE_CHECK_ARG(ctx, "fhe_set_pbs_variant: null context");
-    FHE_CHECK_ARG(variant == FHE_PBS_FP64, "fhe_set_pbs_variant: unknown variant %d", variant);
+    FHE_CHECK_ARG(variant == FHE_PBS_FP64 || variant == FHE_PBS_MFMA_SPLIT,
+                  "fhe_set_pbs_variant: unknown variant %d", variant);
     ctx->variant = variant;
     return FHE_OK;
 }
@@ -110,6 +151,7 @@ int fhe_load_bk(fhe_ctx *ctx, int level, const void *bk_host, int format)
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(tmp);
     HIP_TRY(e, "fhe_load_bk");
+    if (l1) ctx->bk1m_valid = false;
     return FHE_OK;
 }
 
@@ -135,10 +177,7 @@ int fhe_blind_rotate_lvl01_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint3
         set_error("fhe_blind_rotate_lvl01_dev: lvl01 bootstrapping key not loaded");
         return FHE_ERR_NOT_LOADED;
     }
-    HIP_TRY(launch_blind_rotate1(in_dev, int(ctx->p.n0), ctx->bk1, tv_dev, tv_stride, acc_dev,
-                                 false, B, pick(ctx, stream)),
-            "blind rotate lvl01");
-    return FHE_OK;
+    return launch_pbs1(ctx, in_dev, tv_dev, tv_stride, acc_dev, false, B, pick(ctx, stream));
 }
 
 int fhe_pbs_lvl01_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint32_t *tv_dev, int tv_stride,
@@ -151,10 +190,7 @@ int fhe_pbs_lvl01_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint32_t *tv_d
         set_error("fhe_pbs_lvl01_dev: lvl01 bootstrapping key not loaded");
         return FHE_ERR_NOT_LOADED;
     }
-    HIP_TRY(launch_blind_rotate1(in_dev, int(ctx->p.n0), ctx->bk1, tv_dev, tv_stride, out_dev,
-                                 true, B, pick(ctx, stream)),
-            "pbs lvl01");
-    return FHE_OK;
+    return launch_pbs1(ctx, in_dev, tv_dev, tv_stride, out_dev, true, B, pick(ctx, stream));
 }
 
 int fhe_pbs_lvl02_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint64_t *tv_dev, int tv_stride,
@@ -266,6 +302,20 @@ int fhe_ifft_lvl1_dev(fhe_ctx *ctx, const uint32_t *poly_dev, double *fd_dev, in
     return FHE_OK;
 }
 
+int fhe_fwd_digits_lvl1_mfma_dev(fhe_ctx *ctx, const int32_t *poly_dev, double *fd_dev, int B,
+                                 void *stream)
+{
+    FHE_CHECK_ARG(ctx && poly_dev && fd_dev && B >= 0, "fhe_fwd_digits_lvl1_mfma_dev: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    hipStream_t s = pick(ctx, stream);
+    if (!ctx->mfma_tab) {
+        HIP_TRY(hipMalloc(&ctx->mfma_tab, mfma_tables_bytes()), "MFMA table allocation");
+        HIP_TRY(launch_mfma_tables(ctx->mfma_tab, s), "MFMA tables");
+    }
+    HIP_TRY(launch_fwd_digits_mfma(poly_dev, ctx->mfma_tab, fd_dev, B, s), "split-MFMA transform");
+    return FHE_OK;
+}
+
 int fhe_fft_lvl1_dev(fhe_ctx *ctx, const double *fd_dev, uint32_t *poly_dev, int B, void *stream)
 {
     FHE_CHECK_ARG(ctx && poly_dev && fd_dev && B >= 0, "fhe_fft_lvl1_dev: bad argument");
@@ -319,9 +369,8 @@ int run_bootstrap(fhe_ctx *ctx, int in_level, const void *in, int out_level, voi
         HIP_TRY(tv.ide ? launch_tv_ide<uint32_t>(ctx->tv1, uint32_t(tv.value), Lvl1::N, s)
                        : launch_tv_const<uint32_t>(ctx->tv1, uint32_t(tv.value), Lvl1::N, s),
                 "test vector");
-        HIP_TRY(launch_blind_rotate1(ctx->tmp0, n0, ctx->bk1, ctx->tv1, 0,
-                                     static_cast<uint32_t *>(out), true, B, s),
-                "pbs lvl01");
+        rc = launch_pbs1(ctx, ctx->tmp0, ctx->tv1, 0, static_cast<uint32_t *>(out), true, B, s);
+        if (rc) return rc;
         if (post)
             HIP_TRY(launch_tlwe_add_body<uint32_t>(static_cast<uint32_t *>(out), Lvl1::N + 1,
                                                    uint32_t(post), B, s),

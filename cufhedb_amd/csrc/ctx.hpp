@@ -20,6 +20,11 @@ struct fhe_ctx {
     fhe::cd *trgsw1 = nullptr;  // external-product TRGSW in device layout
     fhe::cd *trgsw2 = nullptr;
     int variant = FHE_PBS_FP64;
+    // split-MFMA variant: FP32 key in its frequency order (rebuilt from bk1
+    // on demand, bk1m_valid cleared by every lvl1 key upload) and its tables
+    float *bk1m = nullptr;
+    bool bk1m_valid = false;
+    void *mfma_tab = nullptr;
     // grow-on-demand scratch for the bootstrap operators
     uint32_t *tv1 = nullptr;     // one lvl1 test vector
     uint64_t *tv2 = nullptr;     // one lvl2 test vector

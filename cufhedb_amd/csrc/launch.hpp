@@ -26,6 +26,18 @@ hipError_t launch_fft1(const double *fd, uint32_t *poly, int B, hipStream_t s);
 hipError_t launch_sample_extract1(const uint32_t *trlwe, uint32_t *tlwe, int index, int B,
                                   hipStream_t s);
 
+// ---- lvl1 split-MFMA variant, pbs_lvl1_mfma.hip -------------------------
+size_t mfma_tables_bytes();
+size_t bk1_mfma_floats(int n0);
+hipError_t launch_mfma_tables(void *tab, hipStream_t s);
+// FP64 device-layout key -> FP32 key in the variant's frequency order
+hipError_t launch_bk1_to_mfma(const cd *bk_dev, float *bkm, int n0, hipStream_t s);
+hipError_t launch_fwd_digits_mfma(const int32_t *poly, const void *tab, double *fd, int B,
+                                  hipStream_t s);
+hipError_t launch_blind_rotate1_mfma(const uint32_t *in, int n0, const float *bkm, const void *tab,
+                                     const uint32_t *tv, int tv_stride, uint32_t *out,
+                                     bool sample_extract, int B, hipStream_t s);
+
 // ---- lvl2 (N = 2048, 64-bit torus), pbs_lvl2.hip ------------------------
 size_t bk2_device_elems(int n0);
 hipError_t launch_bk2_from_int(const uint64_t *bk_int, cd *bk_dev, int n0, hipStream_t s);

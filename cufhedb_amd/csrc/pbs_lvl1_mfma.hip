@@ -1,0 +1,441 @@
+// pbs_lvl1_mfma.hip -- lvl1 blind rotation with the forward (digit)
+// transforms on the fp16 matrix cores: the "split MFMA" variant.
+//
+// Same blind rotation, decomposition, rotation and sample extraction as
+// pbs_lvl1.hip (reference: thirdparty/TFHEpp/include/gatebootstrapping.hpp:15-43,
+// src/detwfa.cpp:21-30, src/trgsw.cpp:9-103; GPU counterpart
+// src/cuHEDB/fft_gpu/BlindRotate_gpu.cuh:401-531).  What changes is how the
+// six digit polynomials of every CMUX reach the frequency domain:
+//
+//   Z[m2 + 16 m1] = sum_n1 W32^{n1 m1} T(n1, m2) sum_n2 A[m2][n2] z[n1 + 32 n2]
+//     A[m2][n2] = exp(i pi n2 (4 m2 + 1) / 32)      (radix-16 stage + twist)
+//     T(n1, m2) = exp(i pi n1 (4 m2 + 1) / 1024)    (inter-stage twiddle)
+//
+// which is the twisted negacyclic map Z[m] = sum_n z_n w^{n(4m+1)} of the FP64
+// path, factored as n = n1 + 32 n2, m = m2 + 16 m1.  Both stages are real
+// matrix products on v_mfma_f32_*_f16 with FP32 accumulation:
+//   stage A  32x32x16: the digits (|d| <= 32, exact in fp16) times the
+//            matrix A split in two fp16 limbs (hi + lo, ~22 bits): 4 MFMAs;
+//   stage B  16x16x32: the twiddled stage-A output (FP32) and W32, both
+//            split hi + lo, products hi*hi + hi*lo + lo*hi: 24 MFMAs.
+// The data never leaves the wave: stage A consumes the digits in the lane
+// layout the rotation produces (lane L holds z[L + 64 j]); one LDS
+// transpose (16 x 34 float2, conflict-free) feeds stage B.  The key is held
+// in FP32 in stage B's output layout, the frequency-domain multiply-
+// accumulate is FP32, and the two inverse transforms stay FP64
+// (fft512_inv), fed through one conflict-free LDS permutation each.
+// Relative error of the forward transform: about 2^-23 (model and tests);
+// the PBS output phase error it adds is measured against the FP64 path.
+#include "fft64.hpp"
+#include "launch.hpp"
+#include "params.hpp"
+
+namespace fhe {
+
+namespace {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int N1 = Lvl1::N;
+constexpr uint32_t kDecOff1m = decomp_total_offset<uint32_t, 32, Lvl1::l, Lvl1::Bgbit>();
+constexpr int kStrideB = 34;  // float2 row stride of the stage-A -> stage-B transpose
+
+// Per-lane constants of the two stages (built once on the device).
+struct MfmaTables {
+    half8 a[2][2][64];        // [limb][K step][lane]   stage-A matrix, A operand
+    half8 w[2][2][2][64];     // [re/im][limb][tile][lane]  W32, B operand (tile 1 unused)
+    float2 t[64][3];          // [lane]: T(n1, 2h), w^{4 n1}, w^{16 n1}  twiddle bases
+};
+
+__device__ __forceinline__ uint32_t rot_minus_one_m(const uint32_t *poly, uint32_t a, int n)
+{
+    uint32_t idx = (uint32_t(n) - a) & (2u * N1 - 1);
+    uint32_t v = poly[idx & (N1 - 1)];
+    if (idx & N1) v = 0u - v;
+    return v - poly[n];
+}
+
+// signed gadget digit d (DecompositionPolynomial, trgsw.cpp:21-39)
+__device__ __forceinline__ int digit1m(uint32_t x, int d)
+{
+    constexpr uint32_t flip = (1u << 31) | (1u << 25) | (1u << 19);
+    const uint32_t u = (x + kDecOff1m) ^ flip;
+    return int32_t(u << (Lvl1::Bgbit * d)) >> (32 - Lvl1::Bgbit);
+}
+
+// conflict-free slot of frequency m in the 512-entry float2 permutation buffer
+__device__ __forceinline__ int perm_slot(int m)
+{
+    return m ^ (((m >> 3) & 7) << 2) ^ ((m >> 6) & 3);
+}
+
+// negate elements 1, 3, 5, 7 (the high half of every dword)
+__device__ __forceinline__ half8 neg_odd(half8 x)
+{
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 u = __builtin_bit_cast(u32x4, x);
+    u ^= 0x80008000u & 0xFFFF0000u;
+    return __builtin_bit_cast(half8, u);
+}
+
+__device__ __forceinline__ void split2(float x, _Float16 &hi, _Float16 &lo)
+{
+    hi = _Float16(x);
+    lo = _Float16(x - float(hi));
+}
+
+__global__ void k_mfma_tables(MfmaTables *tab)
+{
+    const int lane = threadIdx.x;  // one block of 64
+    for (int s = 0; s < 2; s++) {
+        const int i = lane & 31;
+        const int t = i >> 3, hr = (i >> 2) & 1, q = i & 3;
+        const int m2 = 4 * t + 2 * hr + (q >> 1), eo = q & 1;
+        for (int e = 0; e < 8; e++) {
+            const int n2 = 2 * (4 * s + (e >> 1)) + (lane >> 5), ei = e & 1;
+            double sv, cv;
+            sincospi(double(n2 * (4 * m2 + 1)) / 32.0, &sv, &cv);
+            const double val = eo == 0 ? (ei == 0 ? cv : -sv) : (ei == 0 ? sv : cv);
+            const _Float16 hi = _Float16(val);
+            tab->a[0][s][lane][e] = hi;
+            tab->a[1][s][lane][e] = _Float16(val - double(hi));
+        }
+    }
+    for (int tile = 0; tile < 2; tile++)
+        for (int e = 0; e < 8; e++) {
+            const int n1 = 8 * (lane >> 4) + e, m1 = 16 * tile + (lane & 15);
+            double sv, cv;
+            sincospi(double((n1 * m1) & 31) / 16.0, &sv, &cv);
+            const _Float16 rh = _Float16(cv), ih = _Float16(sv);
+            tab->w[0][0][tile][lane][e] = rh;
+            tab->w[0][1][tile][lane][e] = _Float16(cv - double(rh));
+            tab->w[1][0][tile][lane][e] = ih;
+            tab->w[1][1][tile][lane][e] = _Float16(sv - double(ih));
+        }
+    // T(n1, m2) = w^{n1 (4 m2 + 1)}, w = exp(i pi / 1024); a lane needs
+    // m2 = 4 tt + 2 h + u: T(n1, 2h) times powers of w^{4 n1} (u) and w^{16 n1} (tt)
+    const int n1 = lane & 31, h = lane >> 5;
+    const int ex[3] = {n1 * (8 * h + 1), 4 * n1, 16 * n1};
+    for (int k = 0; k < 3; k++) {
+        double sv, cv;
+        sincospi(double(ex[k]) / 1024.0, &sv, &cv);
+        tab->t[lane][k] = float2{float(cv), float(sv)};
+    }
+}
+
+// FP64 device-layout key (lane s, register q <-> frequency fd_index1024(s, q),
+// already scaled by 1/512) -> FP32 in stage-B output order:
+// [i][row][col][k2 < 4][lane][2 complex], element k = 2 k2 + e of lane l is
+// frequency 4 (l >> 4) + (k & 3) + 16 (l & 15) + 256 (k >> 2).
+__global__ void k_bk1_to_mfma(const cd *__restrict__ src, float2 *__restrict__ dst, int npoly)
+{
+    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= size_t(npoly) * 512) return;
+    const size_t p = gid / 512;
+    const int rem = int(gid % 512);
+    const int k2 = rem >> 7, lane = (rem >> 1) & 63, e = rem & 1;
+    const int k = 2 * k2 + e;
+    const int m = 4 * (lane >> 4) + (k & 3) + 16 * (lane & 15) + 256 * (k >> 2);
+    const int q = m >> 6, r = m & 63;
+    const int s = (r & 7) * 8 + (r >> 3);  // fd_index1024(s, q) == m
+    const cd v = src[p * 512 + q * 64 + s];
+    dst[gid] = float2{float(v.x), float(v.y)};
+}
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b)
+{
+    return float2{__fmaf_rn(a.x, b.x, -a.y * b.y), __fmaf_rn(a.x, b.y, a.y * b.x)};
+}
+
+__device__ __forceinline__ float2 fmac2(float2 acc, float2 a, float2 b)
+{
+    acc.x = __fmaf_rn(a.x, b.x, acc.x);
+    acc.x = __fmaf_rn(-a.y, b.y, acc.x);
+    acc.y = __fmaf_rn(a.x, b.y, acc.y);
+    acc.y = __fmaf_rn(a.y, b.x, acc.y);
+    return acc;
+}
+
+// Forward split-MFMA transform of one digit polynomial held as stage-A B
+// operands (b[s] element e = re/im of z[lane + 64 (4 s + e / 2)]), result as
+// two 16x16 output tiles: lane l, tile t, element q holds frequency
+// 4 (l >> 4) + q + 16 (l & 15) + 256 t.  buf: 16 x 34 float2 of LDS.
+struct MfmaConsts {
+    half8 ah[2], al[2];     // stage-A matrix limbs
+    float2 tb, t4, t16;     // twiddle bases
+};
+
+__device__ __forceinline__ MfmaConsts load_consts(const MfmaTables *__restrict__ tab, int lane)
+{
+    MfmaConsts mc;
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        mc.ah[s] = tab->a[0][s][lane];
+        mc.al[s] = tab->a[1][s][lane];
+    }
+    mc.tb = tab->t[lane][0];
+    mc.t4 = tab->t[lane][1];
+    mc.t16 = tab->t[lane][2];
+    return mc;
+}
+
+__device__ __forceinline__ void mfma_fwd(const half8 b[2], const MfmaConsts &mc,
+                                         const MfmaTables *__restrict__ tab, float2 *buf, int lane,
+                                         f32x4 zr[2], f32x4 zi[2])
+{
+    const int n1 = lane & 31, h = lane >> 5;
+    f32x16 C = {};
+    C = __builtin_amdgcn_mfma_f32_32x32x16_f16(mc.ah[0], b[0], C, 0, 0, 0);
+    C = __builtin_amdgcn_mfma_f32_32x32x16_f16(mc.al[0], b[0], C, 0, 0, 0);
+    C = __builtin_amdgcn_mfma_f32_32x32x16_f16(mc.ah[1], b[1], C, 0, 0, 0);
+    C = __builtin_amdgcn_mfma_f32_32x32x16_f16(mc.al[1], b[1], C, 0, 0, 0);
+    // inter-stage twiddle, then transpose Y'[m2][n1] through LDS
+    {
+        float2 tw = mc.tb;
+#pragma unroll
+        for (int tt = 0; tt < 4; tt++) {
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const float2 tk = u ? cmulf(tw, mc.t4) : tw;
+                const float2 y = cmulf(float2{C[4 * tt + 2 * u], C[4 * tt + 2 * u + 1]}, tk);
+                buf[(4 * tt + 2 * h + u) * kStrideB + n1] = y;
+            }
+            if (tt < 3) tw = cmulf(tw, mc.t16);
+        }
+    }
+    __syncthreads();
+    half8 yrh, yrl, yih, yil, nyih, nyil;
+    {
+        const float4 *row =
+            reinterpret_cast<const float4 *>(buf + (lane & 15) * kStrideB + 8 * (lane >> 4));
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const float4 x = row[p];
+            _Float16 hi, lo;
+            split2(x.x, hi, lo); yrh[2 * p] = hi; yrl[2 * p] = lo;
+            split2(x.y, hi, lo); yih[2 * p] = hi; yil[2 * p] = lo;
+            split2(x.z, hi, lo); yrh[2 * p + 1] = hi; yrl[2 * p + 1] = lo;
+            split2(x.w, hi, lo); yih[2 * p + 1] = hi; yil[2 * p + 1] = lo;
+        }
+    }
+    __syncthreads();
+    nyih = -yih;
+    nyil = -yil;
+    // W32 columns 16..31 equal columns 0..15 times (-1)^{n1}: tile 1 runs on
+    // tile 0's matrix with the odd-n1 (odd-element) inputs negated
+    const half8 wrh = tab->w[0][0][0][lane], wrl = tab->w[0][1][0][lane];
+    const half8 wih = tab->w[1][0][0][lane], wil = tab->w[1][1][0][lane];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        if (t == 1) {
+            yrh = neg_odd(yrh); yrl = neg_odd(yrl);
+            yih = neg_odd(yih); yil = neg_odd(yil);
+            nyih = neg_odd(nyih); nyil = neg_odd(nyil);
+        }
+        f32x4 a = {}, c = {};
+        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(yrh, wrh, a, 0, 0, 0);
+        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(yrh, wrl, a, 0, 0, 0);
+        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(yrl, wrh, a, 0, 0, 0);
+        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(nyih, wih, a, 0, 0, 0);
+        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(nyih, wil, a, 0, 0, 0);
+        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(nyil, wih, a, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(yrh, wih, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(yrh, wil, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(yrl, wih, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(yih, wrh, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(yih, wrl, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(yil, wrh, c, 0, 0, 0);
+        zr[t] = a;
+        zi[t] = c;
+    }
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+k_blind_rotate1_mfma(const uint32_t *__restrict__ in, int n0, const float4 *__restrict__ bkm,
+                     const MfmaTables *__restrict__ tab, const uint32_t *__restrict__ tv,
+                     int tv_stride, uint32_t *__restrict__ out, int sample_extract, int B)
+{
+    __shared__ uint32_t acc[2 * N1];
+    __shared__ cd tile[kFftLdsElems];
+    const int lane = threadIdx.x;
+    const int ct = blockIdx.x;
+    if (ct >= B) return;
+    const uint32_t *c0 = in + size_t(ct) * (n0 + 1);
+    const uint32_t *tvp = tv + size_t(ct) * tv_stride;
+    float2 *buf = reinterpret_cast<float2 *>(tile);  // 576 x 16 B >= 544 / 512 float2 needed
+
+    const MfmaConsts mc = load_consts(tab, lane);
+    const Twiddles1024 tw64 = make_twiddles1024(lane);
+
+    const uint32_t bbar = 2u * N1 - (c0[n0] >> (32 - 1 - Lvl1::nbit));
+#pragma unroll
+    for (int k = 0; k < N1 / 64; k++) {
+        const int n = lane + 64 * k;
+        acc[n] = 0;
+        uint32_t idx = (uint32_t(n) - bbar) & (2u * N1 - 1);
+        uint32_t v = tvp[idx & (N1 - 1)];
+        acc[N1 + n] = (idx & N1) ? 0u - v : v;
+    }
+    __syncthreads();
+
+    constexpr uint32_t kRound = 1u << (32 - 2 - Lvl1::nbit);
+    for (int i = 0; i < n0; i++) {
+        const uint32_t abar = (c0[i] + kRound) >> (32 - 1 - Lvl1::nbit);
+        if (abar == 0) continue;
+        const float4 *bki = bkm + size_t(i) * Lvl1::rows * 2 * 256;
+        float2 fd0[8], fd1[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) fd0[k] = fd1[k] = float2{0.f, 0.f};
+#pragma unroll 1
+        for (int r = 0; r < Lvl1::rows; r++) {
+            const int c = r / Lvl1::l, d = r - c * Lvl1::l;
+            const uint32_t *poly = acc + c * N1;
+            // digits as stage-A B operands: step s holds j = 4s..4s+3 (re, im)
+            half8 b[2];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int n = lane + 64 * j;
+                b[j >> 2][2 * (j & 3)] = _Float16(digit1m(rot_minus_one_m(poly, abar, n), d));
+                b[j >> 2][2 * (j & 3) + 1] =
+                    _Float16(digit1m(rot_minus_one_m(poly, abar, n + 512), d));
+            }
+            f32x4 zr[2], zi[2];
+            mfma_fwd(b, mc, tab, buf, lane, zr, zi);
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                // this tile's key entries (both output columns), FP32
+                float4 k0[2], k1[2];
+#pragma unroll
+                for (int k2 = 0; k2 < 2; k2++) {
+                    k0[k2] = bki[(r * 2 + 0) * 256 + (2 * t + k2) * 64 + lane];
+                    k1[k2] = bki[(r * 2 + 1) * 256 + (2 * t + k2) * 64 + lane];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int k = 4 * t + q;  // frequency 4 (lane>>4) + q + 16 (lane&15) + 256 t
+                    const float2 z{zr[t][q], zi[t][q]};
+                    const float4 p0 = k0[q >> 1], p1 = k1[q >> 1];
+                    const float2 key0 = (k & 1) ? float2{p0.z, p0.w} : float2{p0.x, p0.y};
+                    const float2 key1 = (k & 1) ? float2{p1.z, p1.w} : float2{p1.x, p1.y};
+                    fd0[k] = fmac2(fd0[k], z, key0);
+                    fd1[k] = fmac2(fd1[k], z, key1);
+                }
+            }
+        }
+        // both columns: permute to the FP64 transform's layout, invert, accumulate
+#pragma unroll 1
+        for (int c = 0; c < 2; c++) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int m = 4 * (lane >> 4) + (k & 3) + 16 * (lane & 15) + 256 * (k >> 2);
+                buf[perm_slot(m)] = c == 0 ? fd0[k] : fd1[k];
+            }
+            __syncthreads();
+            cd v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const float2 f = buf[perm_slot(fd_index1024(lane, q))];
+                v[q] = cd{double(f.x), double(f.y)};
+            }
+            __syncthreads();
+            fft512_inv(v, tile, lane, tw64);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int n = c * N1 + lane + 64 * j;
+                acc[n] += trunc_torus32(v[j].x);
+                acc[n + 512] += trunc_torus32(v[j].y);
+            }
+        }
+        __syncthreads();
+    }
+
+    if (sample_extract) {
+        uint32_t *o = out + size_t(ct) * (N1 + 1);
+#pragma unroll
+        for (int k = 0; k < N1 / 64; k++) {
+            const int n = lane + 64 * k;
+            o[n] = n == 0 ? acc[0] : 0u - acc[N1 - n];
+        }
+        if (lane == 0) o[N1] = acc[N1];
+    } else {
+        uint32_t *o = out + size_t(ct) * 2 * N1;
+#pragma unroll
+        for (int k = 0; k < 2 * N1 / 64; k++) o[lane + 64 * k] = acc[lane + 64 * k];
+    }
+}
+
+// The split-MFMA forward transform alone: B small-integer polynomials
+// (|x| <= 2048, exact in fp16; gadget digits are |x| <= 32) -> TFHEpp FD
+// layout (spqlios order, like k_ifft1), for parity tests of the transform.
+__global__ __launch_bounds__(64) void k_fwd_digits_mfma(const int32_t *__restrict__ poly,
+                                                         const MfmaTables *__restrict__ tab,
+                                                         double *__restrict__ fd, int B)
+{
+    __shared__ float2 buf[16 * kStrideB];
+    const int lane = threadIdx.x;
+    const int p = blockIdx.x;
+    if (p >= B) return;
+    const MfmaConsts mc = load_consts(tab, lane);
+    const int32_t *a = poly + size_t(p) * N1;
+    half8 b[2];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        b[j >> 2][2 * (j & 3)] = _Float16(a[lane + 64 * j]);
+        b[j >> 2][2 * (j & 3) + 1] = _Float16(a[lane + 64 * j + 512]);
+    }
+    f32x4 zr[2], zi[2];
+    mfma_fwd(b, mc, tab, buf, lane, zr, zi);
+    double *o = fd + size_t(p) * N1;
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int m = 4 * (lane >> 4) + q + 16 * (lane & 15) + 256 * t;
+            o[rev9(m)] = zr[t][q];
+            o[rev9(m) + 512] = zi[t][q];
+        }
+}
+
+}  // namespace
+
+hipError_t launch_fwd_digits_mfma(const int32_t *poly, const void *tab, double *fd, int B,
+                                  hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fwd_digits_mfma, dim3(B), dim3(64), 0, s, poly,
+                       static_cast<const MfmaTables *>(tab), fd, B);
+    return hipGetLastError();
+}
+
+size_t mfma_tables_bytes() { return sizeof(MfmaTables); }
+size_t bk1_mfma_floats(int n0) { return size_t(n0) * Lvl1::rows * 2 * 512 * 2; }
+
+hipError_t launch_mfma_tables(void *tab, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mfma_tables, dim3(1), dim3(64), 0, s, static_cast<MfmaTables *>(tab));
+    return hipGetLastError();
+}
+
+hipError_t launch_bk1_to_mfma(const cd *bk_dev, float *bkm, int n0, hipStream_t s)
+{
+    const int npoly = n0 * Lvl1::rows * 2;
+    const size_t n = size_t(npoly) * 512;
+    hipLaunchKernelGGL(k_bk1_to_mfma, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, bk_dev,
+                       reinterpret_cast<float2 *>(bkm), npoly);
+    return hipGetLastError();
+}
+
+hipError_t launch_blind_rotate1_mfma(const uint32_t *in, int n0, const float *bkm, const void *tab,
+                                     const uint32_t *tv, int tv_stride, uint32_t *out,
+                                     bool sample_extract, int B, hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_blind_rotate1_mfma, dim3(B), dim3(64), 0, s, in, n0,
+                       reinterpret_cast<const float4 *>(bkm), static_cast<const MfmaTables *>(tab),
+                       tv, tv_stride, out, int(sample_extract), B);
+    return hipGetLastError();
+}
+
+}  // namespace fhe

@@ -124,7 +124,11 @@ int fhe_load_ksk(fhe_ctx *ctx, int kind, const uint32_t *ksk_host);
 
 /* PBS kernel variant for the lvl1 blind rotation. */
 enum fhe_pbs_variant {
-    FHE_PBS_FP64 = 0 /* FP64 negacyclic FFT, LDS-resident (plain method) */
+    FHE_PBS_FP64 = 0,      /* FP64 negacyclic FFT, LDS-resident (plain method) */
+    FHE_PBS_MFMA_SPLIT = 1 /* forward digit transforms as radix-16 x 32 DFT matrix
+                              products on fp16 MFMA (hi/lo limbs, FP32
+                              accumulation), FP32 frequency-domain MAC, FP64
+                              inverse transforms; lvl1 PBS only */
 };
 int fhe_set_pbs_variant(fhe_ctx *ctx, int variant);
 
@@ -177,6 +181,12 @@ int fhe_external_product_lvl2_dev(fhe_ctx *ctx, const uint64_t *trlwe_dev,
  * B polynomials per call. */
 int fhe_ifft_lvl1_dev(fhe_ctx *ctx, const uint32_t *poly_dev, double *fd_dev, int B, void *stream);
 int fhe_fft_lvl1_dev(fhe_ctx *ctx, const double *fd_dev, uint32_t *poly_dev, int B, void *stream);
+/* TwistIFFT (mulfft.hpp:113-121) of small-integer polynomials (|x| <= 2048,
+ * e.g. gadget digits) computed by the split-MFMA variant's transform (fp16
+ * MFMA, hi/lo limbs, FP32 accumulation); TFHEpp FD layout out.  Exposes the
+ * variant's forward transform for parity tests against TwistIFFT. */
+int fhe_fwd_digits_lvl1_mfma_dev(fhe_ctx *ctx, const int32_t *poly_dev, double *fd_dev, int B,
+                                 void *stream);
 int fhe_ifft_lvl2_dev(fhe_ctx *ctx, const uint64_t *poly_dev, double *fd_dev, int B, void *stream);
 int fhe_fft_lvl2_dev(fhe_ctx *ctx, const double *fd_dev, uint64_t *poly_dev, int B, void *stream);
 
