@@ -136,6 +136,21 @@ def run_queries(F, ctx, sk, p, rows, reps):
         F.check(L_.fhe_timer_stop(ctx.h, None, ctypes.byref(ms)))
         return float(ms.value) / reps
 
+    # lvl2 PBS throughput (the chains' dominant operation): batch 4096
+    B2 = 4096
+    c0 = F.tlwe_encrypt_u32(303, 0, sk.s0, (rng.integers(0, 8, B2) << 28).astype(np.uint32),
+                            p.alpha0)
+    d0 = F.DeviceBuffer.from_array(c0)
+    tv2 = F.DeviceBuffer.from_array(np.full(2048, 1 << 61, np.uint64))
+    o2 = F.DeviceBuffer(B2 * 2049 * 8)
+    ms2 = timed(lambda: F.check(L_.fhe_pbs_lvl02_dev(ctx.h, d0.ptr, tv2.ptr, 0, o2.ptr, B2, None)))
+    res["lvl2_pbs"] = {"workload": f"lvl2 PBS (N=2048, l=4, Bgbit=9, 64-bit torus), batch {B2}",
+                       "ms_per_step": ms2, "pbs_per_s": B2 / (ms2 / 1e3)}
+    print(f"bench: lvl2 PBS batch {B2}: {ms2:.1f} ms, {B2 / (ms2 / 1e3):.0f} PBS/s",
+          file=sys.stderr, flush=True)
+    for b in (d0, tv2, o2):
+        b.free()
+
     # range filter x <= t, 20-bit values on the 64-bit torus, shared threshold
     bits = 20
     x = rng.integers(0, 1 << bits, rows)

@@ -120,16 +120,16 @@ __device__ __forceinline__ void fft1024_inv(cd v[8], cd *xch, const cd *pxch, in
 }
 
 // Truncation toward zero of the exact value of x, reduced mod 2^64
-// (execute_direct_torus64's mantissa shift; |x| < 1 gives 0).
+// (execute_direct_torus64's mantissa shift; |x| < 1 gives 0), in FP64
+// arithmetic that is exact for |x| < 2^85: t = trunc(x) = H 2^32 + L with
+// L in [0, 2^32) and H reduced mod 2^32 the same way.
 __device__ __forceinline__ uint64_t trunc_torus64(double x)
 {
-    const uint64_t bits = uint64_t(__double_as_longlong(x));
-    const int expo = int((bits >> 52) & 0x7FF);
-    if (expo < 1023) return 0;
-    const uint64_t mant = (bits & 0x000FFFFFFFFFFFFFull) | 0x0010000000000000ull;
-    const int sh = expo - 1075;
-    uint64_t v = sh >= 64 ? 0ull : sh >= 0 ? (mant << sh) : (mant >> (-sh));
-    return (bits >> 63) ? 0ull - v : v;
+    const double t = trunc(x);
+    const double H = floor(t * 0x1p-32);
+    const double L = __fma_rn(-H, 0x1p32, t);
+    const double Hm = __fma_rn(-floor(H * 0x1p-32), 0x1p32, H);
+    return (uint64_t(uint32_t(Hm)) << 32) | uint64_t(uint32_t(L));
 }
 
 __device__ __forceinline__ uint64_t rot_minus_one2(const uint64_t *poly, uint32_t a, int n)
@@ -140,12 +140,20 @@ __device__ __forceinline__ uint64_t rot_minus_one2(const uint64_t *poly, uint32_
     return v - poly[n];
 }
 
-__device__ __forceinline__ double digit2(uint64_t x, int d)
+// Gadget digits of x (DecompositionPolynomial, trgsw.cpp:21-39, lvl2):
+//   digit_d = ((x + offset) >> (64 - 9(d+1)) & 511) - 256.
+// prep2 adds the offset and flips the top bit of every 9-bit field, so each
+// digit is one signed 9-bit field read: the fields at bits 55, 46 and 37 lie
+// in the high word, the one at bit 28 straddles both words.
+constexpr uint64_t kDigitFlip2 = (1ull << 63) | (1ull << 54) | (1ull << 45) | (1ull << 36);
+__device__ __forceinline__ uint64_t prep2(uint64_t x) { return (x + kDecOff2) ^ kDigitFlip2; }
+__device__ __forceinline__ double digit2p(uint64_t u, int d)
 {
-    const int sh = 64 - Lvl2::Bgbit * (d + 1);
-    return double(int64_t((x + kDecOff2) >> sh & ((1ull << Lvl2::Bgbit) - 1)) -
-                  (int64_t(1) << (Lvl2::Bgbit - 1)));
+    const uint32_t hi = uint32_t(u >> 32), lo = uint32_t(u);
+    const uint32_t top = d < 3 ? hi << (Lvl2::Bgbit * d) : ((hi << 4) | (lo >> 28)) << 23;
+    return double(int32_t(top) >> (32 - Lvl2::Bgbit));
 }
+__device__ __forceinline__ double digit2(uint64_t x, int d) { return digit2p(prep2(x), d); }
 
 // row pointer (both columns) -> this wave's half: [c][h][q][lane]
 __device__ __forceinline__ void load_row2(const cd *row, int lane, int h, cd b0[8], cd b1[8])
@@ -157,7 +165,7 @@ __device__ __forceinline__ void load_row2(const cd *row, int lane, int h, cd b0[
     }
 }
 
-__global__ __launch_bounds__(128) void k_blind_rotate2(const uint32_t *__restrict__ in, int n0,
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_blind_rotate2(const uint32_t *__restrict__ in, int n0,
                                                        const cd *__restrict__ bk,
                                                        const uint64_t *__restrict__ tv,
                                                        int tv_stride, uint64_t *__restrict__ out,
@@ -195,8 +203,6 @@ __global__ __launch_bounds__(128) void k_blind_rotate2(const uint32_t *__restric
         for (int r = 0; r < Lvl2::rows; r++) {
             const int c = r / Lvl2::l, d = r - c * Lvl2::l;
             const uint64_t *poly = acc + c * N2;
-            cd b0[8], b1[8];
-            load_row2(bki + r * kRow2Cd, lane, h, b0, b1);
             cd v[8];
 #pragma unroll
             for (int j = 0; j < 8; j++) {
@@ -205,19 +211,25 @@ __global__ __launch_bounds__(128) void k_blind_rotate2(const uint32_t *__restric
                           digit2(rot_minus_one2(poly, abar, n + 1024), d)};
             }
             fft1024_fwd(v, xch, pxch, lane, h, tw);
+            const cd *row = bki + r * kRow2Cd + h * kHalfCd + lane;
 #pragma unroll
             for (int q = 0; q < 8; q++) {
-                cmac(fd0[q], v[q], b0[q]);
-                cmac(fd1[q], v[q], b1[q]);
+                cmac(fd0[q], v[q], row[q * 64]);
+                cmac(fd1[q], v[q], row[kPoly2Cd + q * 64]);
             }
         }
+        // every rotation read of acc is complete (the transforms synchronised)
         fft1024_inv(fd0, xch, pxch, lane, h, tw);
-        fft1024_inv(fd1, xch, pxch, lane, h, tw);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int n = lane + 64 * j + 512 * h;
             acc[n] += trunc_torus64(fd0[j].x * (1.0 / 1024));
             acc[n + 1024] += trunc_torus64(fd0[j].y * (1.0 / 1024));
+        }
+        fft1024_inv(fd1, xch, pxch, lane, h, tw);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int n = lane + 64 * j + 512 * h;
             acc[N2 + n] += trunc_torus64(fd1[j].x * (1.0 / 1024));
             acc[N2 + n + 1024] += trunc_torus64(fd1[j].y * (1.0 / 1024));
         }

@@ -8,6 +8,10 @@
 //   cuHEDB:: operators               src/cuHEDB/gatebootstrap_gpu.cuh:31-170
 //     MSBGateBootstrapping, IdeGateBootstrapping, HomNOT, HomAND, HomOR,
 //     LOG_to_ARI, log_rescale
+//   cuHEDB:: comparison layer        src/cuHEDB/msb_gpu.cuh:7-193,
+//                                    src/cuHEDB/HomCompare_gpu.cuh:14-76
+//     HomMSB, greater_than, greater_than_equal, less_than, less_than_equal,
+//     equal (+ *_batch forms with an optional broadcast second operand)
 // P is any parameter struct with TFHEpp's member names (n, k, l, Bgbit, T):
 // TFHEpp::lvl1param / lvl2param work unchanged, and cufhedb_amd::lvl1param /
 // lvl2param below are provided for code that does not include TFHEpp.
@@ -336,6 +340,93 @@ inline void MSBGateBootstrapping_batch(cufhedb_amd::Runtime &rt, uint32_t *res, 
                                        bool result_type, int B)
 {
     detail::run_unary(rt, fhe_msb_gate_lvl1_dev, res, in, result_type ? 1 : 0, B);
+}
+
+// ---- comparison layer (msb_gpu.cuh, HomCompare_gpu.cuh) -------------------
+using TLWELvl2 = std::array<uint64_t, cufhedb_amd::lvl2param::n + 1>;
+
+namespace detail {
+template <class T>
+constexpr int level_of() { return sizeof(T) == 4 ? 1 : 2; }
+
+template <class T>
+inline void run_compare(cufhedb_amd::Runtime &rt, int op, const T *a, const T *b, bool b_bcast,
+                        uint32_t *res, uint32_t plain_bits, bool result_type, int B)
+{
+    constexpr int lvl = level_of<T>();
+    const size_t w = lvl == 1 ? cufhedb_amd::lvl1param::n + 1 : cufhedb_amd::lvl2param::n + 1;
+    const size_t in_bytes = size_t(B) * w * sizeof(T);
+    const size_t out_bytes = size_t(B) * (cufhedb_amd::lvl1param::n + 1) * 4;
+    cufhedb_amd::DeviceBuffer da(in_bytes), db(b_bcast ? w * sizeof(T) : in_bytes), dout(out_bytes);
+    da.upload(a, in_bytes);
+    db.upload(b, b_bcast ? w * sizeof(T) : in_bytes);
+    cufhedb_amd::check(fhe_compare_dev(rt.ctx(), lvl, op, da.get(), db.get(), b_bcast ? 1 : 0,
+                                       dout.as<uint32_t>(), plain_bits, result_type ? 1 : 0, B,
+                                       nullptr),
+                       "fhe_compare_dev");
+    cufhedb_amd::check(fhe_ctx_sync(rt.ctx()), "fhe_ctx_sync");
+    dout.download(res, out_bytes);
+}
+}  // namespace detail
+
+// HomMSB (msb_gpu.cuh:51-57 for lvl1, :175-192 for lvl2): result lvl1
+template <class T, size_t W>
+inline void HomMSB(cufhedb_amd::Runtime &rt, TLWELvl1 &res, const std::array<T, W> &tlwe,
+                   uint32_t plain_bits, bool result_type, uint32_t /*stream_id*/ = 0)
+{
+    constexpr int lvl = detail::level_of<T>();
+    cufhedb_amd::DeviceBuffer din(W * sizeof(T)), dout(sizeof(TLWELvl1));
+    din.upload(tlwe.data(), W * sizeof(T));
+    cufhedb_amd::check(fhe_hom_msb_dev(rt.ctx(), lvl, din.get(), dout.as<uint32_t>(), plain_bits,
+                                       result_type ? 1 : 0, 1, nullptr),
+                       "fhe_hom_msb_dev");
+    cufhedb_amd::check(fhe_ctx_sync(rt.ctx()), "fhe_ctx_sync");
+    dout.download(res.data(), sizeof(TLWELvl1));
+}
+
+// greater_than .. equal (HomCompare_gpu.cuh:14-76); the reference's
+// template parameter P becomes the element type of the TLWE arrays
+#define CUFHEDB_AMD_CMP(NAME, OP)                                                             \
+    template <class T, size_t W>                                                              \
+    inline void NAME(cufhedb_amd::Runtime &rt, const std::array<T, W> &c1,                    \
+                     const std::array<T, W> &c2, TLWELvl1 &res, uint32_t plain_bits,          \
+                     bool result_type, uint32_t /*stream_id*/ = 0)                            \
+    {                                                                                         \
+        detail::run_compare(rt, OP, c1.data(), c2.data(), false, res.data(), plain_bits,      \
+                            result_type, 1);                                                  \
+    }                                                                                         \
+    template <class T>                                                                        \
+    inline void NAME##_batch(cufhedb_amd::Runtime &rt, const T *c1, const T *c2,              \
+                             bool c2_broadcast, uint32_t *res, uint32_t plain_bits,           \
+                             bool result_type, int B)                                         \
+    {                                                                                         \
+        detail::run_compare(rt, OP, c1, c2, c2_broadcast, res, plain_bits, result_type, B);   \
+    }
+CUFHEDB_AMD_CMP(greater_than, FHE_CMP_GT)
+CUFHEDB_AMD_CMP(greater_than_equal, FHE_CMP_GE)
+CUFHEDB_AMD_CMP(less_than, FHE_CMP_LT)
+CUFHEDB_AMD_CMP(less_than_equal, FHE_CMP_LE)
+CUFHEDB_AMD_CMP(equal, FHE_CMP_EQ)
+#undef CUFHEDB_AMD_CMP
+
+// LOG_to_ARI / log_rescale (gatebootstrap_gpu.cuh:155-170, HEDB CPU test vector)
+inline void LOG_to_ARI(cufhedb_amd::Runtime &rt, TLWELvl1 &res, const TLWELvl1 &tlwe,
+                       uint32_t /*stream_id*/ = 0)
+{
+    detail::run_unary(
+        rt, [](fhe_ctx *c, const uint32_t *i, uint32_t *o, int, int B, void *s) {
+            return fhe_log_to_ari_lvl1_dev(c, i, o, B, s);
+        },
+        res.data(), tlwe.data(), 0, 1);
+}
+inline void log_rescale(cufhedb_amd::Runtime &rt, TLWELvl1 &res, const TLWELvl1 &tlwe,
+                        uint32_t scale_bits, uint32_t /*stream_id*/ = 0)
+{
+    detail::run_unary(
+        rt, [](fhe_ctx *c, const uint32_t *i, uint32_t *o, int sb, int B, void *s) {
+            return fhe_log_rescale_lvl1_dev(c, i, o, uint32_t(sb), B, s);
+        },
+        res.data(), tlwe.data(), int(scale_bits), 1);
 }
 
 }  // namespace cuHEDB

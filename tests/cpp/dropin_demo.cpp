@@ -70,6 +70,23 @@ int main()
         cuHEDB::HomNOT<lvl1param>(r, ca);
         bad += (int32_t(phase(r.data(), s1.data(), 1024)) > 0) == bool(x);
     }
+    // cuHEDB::greater_than / less_than_equal / equal on 6-bit lvl1 values
+    // (HomCompare_gpu.cuh:14-76; values at scale 2^(32 - 6 - 1), as in
+    // test/HE3_Query/HomComp.cu:75-80) and LOG_to_ARI
+    for (int t = 0; t < 6; t++) {
+        const uint32_t bits = 6;
+        const uint32_t x = gen() % 64, y = t == 0 ? x : gen() % 64;
+        const uint32_t mx = x << (31 - bits), my = y << (31 - bits);
+        cuHEDB::TLWELvl1 cx, cy, r;
+        cufhedb_amd::check(fhe_tlwe_encrypt_u32(14, t, s1.data(), 1024, &mx, p.alpha1, cx.data(), 1), "enc");
+        cufhedb_amd::check(fhe_tlwe_encrypt_u32(15, t, s1.data(), 1024, &my, p.alpha1, cy.data(), 1), "enc");
+        cuHEDB::greater_than(rt, cx, cy, r, bits, cuHEDB::LOGIC, 0);
+        bad += (int32_t(phase(r.data(), s1.data(), 1024)) > 0) != (x > y);
+        cuHEDB::less_than_equal(rt, cx, cy, r, bits, cuHEDB::ARITHMETIC, 0);
+        bad += ((phase(r.data(), s1.data(), 1024) + (1u << 30)) >> 31) != uint32_t(x <= y);
+        cuHEDB::equal(rt, cx, cy, r, bits, cuHEDB::LOGIC, 0);
+        bad += (int32_t(phase(r.data(), s1.data(), 1024)) > 0) != (x == y);
+    }
     std::printf("dropin_demo: %s (%d mismatches)\n", bad ? "FAILED" : "ok", bad);
     return bad ? 1 : 0;
 }
