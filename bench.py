@@ -225,6 +225,9 @@ def main():
                     help="skip the encrypted-query pipelines (range filter, TPC-H Q1 filter)")
     ap.add_argument("--query-rows", type=int, default=1 << 14)
     ap.add_argument("--query-reps", type=int, default=2)
+    ap.add_argument("--mixed", type=int, default=0, metavar="NBATCHES",
+                    help="also run a mixed lvl1-PBS / lvl2-compare batch stream of this many "
+                         "batches (sizes lognormal around 2048, 70%% lvl1)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a plain copy "
                          "kernel of known size, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -349,6 +352,12 @@ def main():
     }
     res["queries"] = (None if args.no_query else
                       run_queries(F, ctx, sk, p, args.query_rows, args.query_reps))
+    if args.mixed:
+        from cufhedb_amd import mixed
+        if args.no_query:  # the stream needs the operator-layer keys
+            run_queries(F, ctx, sk, p, 256, 1)
+        _, res["mixed_stream"] = mixed.run(ctx, sk, p, mixed.make_plan(2026, args.mixed))
+        print(f"bench: mixed stream: {res['mixed_stream']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(threads)
