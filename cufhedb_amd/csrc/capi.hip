@@ -53,6 +53,22 @@ int launch_pbs1(fhe_ctx *ctx, const uint32_t *in, const uint32_t *tv, int tv_str
     return FHE_OK;
 }
 
+// lvl02 key given in TFHEpp's FD form: back to torus polynomials (TwistFFT,
+// FP64, as exact as the FD values allow), then into the blind rotation's
+// layout through its own transform
+hipError_t bk2_fft_to_q(const double *bk_fft, cd *bk_dev, int n0, hipStream_t s)
+{
+    const int npoly = n0 * Lvl2::rows * 2;
+    uint64_t *polys = nullptr;
+    hipError_t e = hipMalloc(&polys, size_t(npoly) * Lvl2::N * sizeof(uint64_t));
+    if (e != hipSuccess) return e;
+    e = launch_fft2(bk_fft, polys, npoly, s);
+    if (e == hipSuccess) e = launch_bk2q_from_int(polys, bk_dev, n0, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(polys);
+    return e;
+}
+
 }  // namespace
 
 extern "C" {
@@ -128,7 +144,7 @@ int fhe_load_bk(fhe_ctx *ctx, int level, const void *bk_host, int format)
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     const int n0 = int(ctx->p.n0);
     const bool l1 = level == 1;
-    const size_t elems = l1 ? bk1_device_elems(n0) : bk2_device_elems(n0);
+    const size_t elems = l1 ? bk1_device_elems(n0) : bk2q_device_elems(n0);
     cd *&dst = l1 ? ctx->bk1 : ctx->bk2;
     if (!dst) HIP_TRY(hipMalloc(&dst, elems * sizeof(cd)), "BK allocation");
     const int rows = l1 ? Lvl1::rows : Lvl2::rows;
@@ -145,8 +161,8 @@ int fhe_load_bk(fhe_ctx *ctx, int level, const void *bk_host, int format)
                     : launch_bk1_from_fft(static_cast<double *>(tmp), dst, n0, ctx->stream);
         else
             e = format == FHE_BK_INT
-                    ? launch_bk2_from_int(static_cast<uint64_t *>(tmp), dst, n0, ctx->stream)
-                    : launch_bk2_from_fft(static_cast<double *>(tmp), dst, n0, ctx->stream);
+                    ? launch_bk2q_from_int(static_cast<uint64_t *>(tmp), dst, n0, ctx->stream)
+                    : bk2_fft_to_q(static_cast<double *>(tmp), dst, n0, ctx->stream);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(tmp);
@@ -203,7 +219,7 @@ int fhe_pbs_lvl02_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint64_t *tv_d
         set_error("fhe_pbs_lvl02_dev: lvl02 bootstrapping key not loaded");
         return FHE_ERR_NOT_LOADED;
     }
-    HIP_TRY(launch_blind_rotate2(in_dev, int(ctx->p.n0), ctx->bk2, tv_dev, tv_stride, out_dev, true,
+    HIP_TRY(launch_blind_rotate2q(in_dev, int(ctx->p.n0), ctx->bk2, tv_dev, tv_stride, out_dev, true,
                                  B, pick(ctx, stream)),
             "pbs lvl02");
     return FHE_OK;
@@ -219,7 +235,7 @@ int fhe_blind_rotate_lvl02_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint6
         set_error("fhe_blind_rotate_lvl02_dev: lvl02 bootstrapping key not loaded");
         return FHE_ERR_NOT_LOADED;
     }
-    HIP_TRY(launch_blind_rotate2(in_dev, int(ctx->p.n0), ctx->bk2, tv_dev, tv_stride, acc_dev,
+    HIP_TRY(launch_blind_rotate2q(in_dev, int(ctx->p.n0), ctx->bk2, tv_dev, tv_stride, acc_dev,
                                  false, B, pick(ctx, stream)),
             "blind rotate lvl02");
     return FHE_OK;
@@ -326,7 +342,7 @@ int fhe_fft_lvl1_dev(fhe_ctx *ctx, const double *fd_dev, uint32_t *poly_dev, int
 int fhe_ifft_lvl2_dev(fhe_ctx *ctx, const uint64_t *poly_dev, double *fd_dev, int B, void *stream)
 {
     FHE_CHECK_ARG(ctx && poly_dev && fd_dev && B >= 0, "fhe_ifft_lvl2_dev: bad argument");
-    HIP_TRY(launch_ifft2(poly_dev, fd_dev, B, pick(ctx, stream)), "ifft lvl2");
+    HIP_TRY(launch_ifft2q(poly_dev, fd_dev, B, pick(ctx, stream)), "ifft lvl2");
     return FHE_OK;
 }
 
@@ -379,7 +395,7 @@ int run_bootstrap(fhe_ctx *ctx, int in_level, const void *in, int out_level, voi
         HIP_TRY(tv.ide ? launch_tv_ide<uint64_t>(ctx->tv2, uint32_t(tv.value), Lvl2::N, s)
                        : launch_tv_const<uint64_t>(ctx->tv2, tv.value, Lvl2::N, s),
                 "test vector");
-        HIP_TRY(launch_blind_rotate2(ctx->tmp0, n0, ctx->bk2, ctx->tv2, 0,
+        HIP_TRY(launch_blind_rotate2q(ctx->tmp0, n0, ctx->bk2, ctx->tv2, 0,
                                      static_cast<uint64_t *>(out), true, B, s),
                 "pbs lvl02");
         if (post)

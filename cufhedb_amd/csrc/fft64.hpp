@@ -60,6 +60,16 @@ __device__ __forceinline__ void cmac(cd &acc, cd a, cd b)
 
 constexpr double kR2 = 0.70710678118654752440;  // sqrt(1/2)
 
+// LDS ordering among the lanes of one wave: a wave executes its LDS
+// operations in order, so a compiler barrier with wavefront-scope fences
+// suffices when an LDS area is private to the wave.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // x * (S i)
 template <int S>
 __device__ __forceinline__ cd mul_si(cd a)

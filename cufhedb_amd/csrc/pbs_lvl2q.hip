@@ -1,0 +1,395 @@
+// pbs_lvl2q.hip -- lvl2 (N = 2048, 64-bit torus, l = 4, Bgbit = 9) blind
+// rotation with four waves per ciphertext.
+//
+// Same algorithm as k_blind_rotate2 in pbs_lvl2.hip (reference:
+// thirdparty/TFHEpp/include/gatebootstrapping.hpp:15-43 with lvl02param,
+// src/detwfa.cpp:21-30, src/trgsw.cpp:9-103; GPU counterpart
+// src/cuHEDB/fft_gpu/BlindRotate_gpu.cuh:534-670), laid out for occupancy:
+// each wave holds 4 of the 1024 complex points per lane instead of 8, so a
+// wave needs about half the registers and three waves fit per SIMD (the
+// accumulator, 2 x 2048 x 64-bit = 32 KB, stays one copy per ciphertext in
+// LDS).  The twisted transform Z[m] = sum_n z_n w2^{n(4m+1)},
+// w2 = exp(i pi/2048), factors as
+//   step A  (registers, then one exchange between the four waves):
+//           u_k[n0] = w2^{n0(4k+1)} sum_p c_p z[n0 + 256 p] i^{pk},
+//           c_p = w2^{256 p},  n0 < 256,  k < 4;
+//   step C  (wave k): the plain 256-point DFT of u_k, Z[4 m' + k]:
+//           a radix-4 pass over the registers, then the 64-point DFT over
+//           the lanes as three radix-4 passes with three wave-local LDS
+//           transposes (row strides 64, 20 and 5 double2, conflict-free for
+//           gfx950's 16-lanes-per-cycle b128 accesses).
+// Layouts: time domain wave w, lane L, register p holds z[L + 64 w + 256 p];
+// frequency domain wave k, lane l, register q holds Z[4 m' + k] with
+// m' = (l >> 4) + 4 ((l >> 2) & 3) + 16 (l & 3) + 64 q.  The inverse runs
+// the same steps mirrored with conjugate twiddles (unnormalised, x 1024;
+// the key is stored pre-scaled by 1/1024, exactly).
+#include "fft64.hpp"
+#include "launch.hpp"
+#include "params.hpp"
+
+namespace fhe {
+
+namespace {
+
+constexpr int N2q = Lvl2::N;
+constexpr uint64_t kDecOff2q = decomp_total_offset<uint64_t, 64, Lvl2::l, Lvl2::Bgbit>();
+constexpr int kTileQ = 320;          // double2 per wave tile (strides 64 / 20 / 5)
+constexpr double kKeyScale2 = 1.0 / 1024;
+
+// per-lane twiddle bases
+struct TwQ {
+    cd beta, rho;  // step A: w2^{n0}, w2^{4 n0}, n0 = lane + 64 wave
+    cd t256;       // W256^{lane}
+    cd t64;        // W64^{lane & 15}
+    cd t16;        // W16^{lane & 3}
+};
+
+__device__ __forceinline__ TwQ make_twq(int lane, int wave)
+{
+    TwQ t;
+    const int n0 = lane + 64 * wave;
+    t.beta = unit_root(double(n0), 2048.0);
+    t.rho = unit_root(double(n0), 512.0);
+    t.t256 = unit_root(double(lane), 128.0);
+    t.t64 = unit_root(double(lane & 15), 32.0);
+    t.t16 = unit_root(double(lane & 3), 8.0);
+    return t;
+}
+
+// c_p = exp(i pi p / 8)
+__device__ __forceinline__ cd cq(int p)
+{
+    constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173;
+    return p == 1 ? cd{c1, s1} : p == 2 ? cd{kR2, kR2} : cd{s1, c1};
+}
+
+// v[q] *= base^q (CONJ: conj(base)^q), q = 1..3
+template <bool CONJ>
+__device__ __forceinline__ void twiddle4(cd v[4], cd base)
+{
+    const cd b2 = cmul(base, base);
+    const cd b3 = cmul(b2, base);
+    v[1] = CONJ ? cmulc(v[1], base) : cmul(v[1], base);
+    v[2] = CONJ ? cmulc(v[2], b2) : cmul(v[2], b2);
+    v[3] = CONJ ? cmulc(v[3], b3) : cmul(v[3], b3);
+}
+
+// 64-point DFT over the lanes of a wave for the four register sequences,
+// as three radix-4 passes; tile is this wave's (kTileQ double2)
+__device__ __forceinline__ void lanes64_fwd(cd v[4], cd *tile, int l, const TwQ &tw)
+{
+    // T1: (lane L, reg m1) -> lane (L0 = l & 15, m1 = l >> 4), reg L1
+#pragma unroll
+    for (int m1 = 0; m1 < 4; m1++) tile[m1 * 64 + l] = v[m1];
+    wave_lds_sync();
+#pragma unroll
+    for (int L1 = 0; L1 < 4; L1++) v[L1] = tile[(l >> 4) * 64 + (l & 15) + 16 * L1];
+    wave_lds_sync();
+    dft4<1>(v[0], v[1], v[2], v[3]);
+    twiddle4<false>(v, tw.t64);
+    // T2: (lane (L0, m1), reg q1) -> lane (L00 = l & 3, rest = l >> 2 = 4 m1 + q1), reg L01
+#pragma unroll
+    for (int q1 = 0; q1 < 4; q1++) tile[((l >> 4) * 4 + q1) * 20 + (l & 15)] = v[q1];
+    wave_lds_sync();
+#pragma unroll
+    for (int L01 = 0; L01 < 4; L01++) v[L01] = tile[(l >> 2) * 20 + (l & 3) + 4 * L01];
+    wave_lds_sync();
+    dft4<1>(v[0], v[1], v[2], v[3]);
+    twiddle4<false>(v, tw.t16);
+    // T3: (lane (L00, rest), reg q21) -> lane 4 rest + q21, reg L00
+#pragma unroll
+    for (int q21 = 0; q21 < 4; q21++) tile[((l >> 2) * 4 + q21) * 5 + (l & 3)] = v[q21];
+    wave_lds_sync();
+#pragma unroll
+    for (int L00 = 0; L00 < 4; L00++) v[L00] = tile[l * 5 + L00];
+    wave_lds_sync();
+    dft4<1>(v[0], v[1], v[2], v[3]);
+}
+
+__device__ __forceinline__ void lanes64_inv(cd v[4], cd *tile, int l, const TwQ &tw)
+{
+    dft4<-1>(v[0], v[1], v[2], v[3]);
+#pragma unroll
+    for (int L00 = 0; L00 < 4; L00++) tile[l * 5 + L00] = v[L00];
+    wave_lds_sync();
+#pragma unroll
+    for (int q21 = 0; q21 < 4; q21++) v[q21] = tile[((l >> 2) * 4 + q21) * 5 + (l & 3)];
+    wave_lds_sync();
+    twiddle4<true>(v, tw.t16);
+    dft4<-1>(v[0], v[1], v[2], v[3]);
+#pragma unroll
+    for (int L01 = 0; L01 < 4; L01++) tile[(l >> 2) * 20 + (l & 3) + 4 * L01] = v[L01];
+    wave_lds_sync();
+#pragma unroll
+    for (int q1 = 0; q1 < 4; q1++) v[q1] = tile[((l >> 4) * 4 + q1) * 20 + (l & 15)];
+    wave_lds_sync();
+    twiddle4<true>(v, tw.t64);
+    dft4<-1>(v[0], v[1], v[2], v[3]);
+#pragma unroll
+    for (int L1 = 0; L1 < 4; L1++) tile[(l >> 4) * 64 + (l & 15) + 16 * L1] = v[L1];
+    wave_lds_sync();
+#pragma unroll
+    for (int m1 = 0; m1 < 4; m1++) v[m1] = tile[m1 * 64 + l];
+    wave_lds_sync();
+}
+
+// Forward: v[p] = z[lane + 64 wave + 256 p] in, Z[4 m' + wave] (layout above)
+// out.  xch: the workgroup's 4 x 256 double2 exchange area, which aliases the
+// four wave tiles (tiles = xch + wave * kTileQ).
+__device__ __forceinline__ void fft1024q_fwd(cd v[4], cd *xch, int lane, int wave, const TwQ &tw)
+{
+    v[1] = cmul(v[1], cq(1));
+    v[2] = cmul(v[2], cq(2));
+    v[3] = cmul(v[3], cq(3));
+    dft4<1>(v[0], v[1], v[2], v[3]);
+    {
+        cd g = tw.beta;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = cmul(v[k], g);
+            if (k < 3) g = cmul(g, tw.rho);
+        }
+    }
+    const int n0 = lane + 64 * wave;
+    // the exchange area overlaps the wave tiles: every wave has left its tile
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; k++) xch[k * 256 + n0] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = xch[wave * 256 + lane + 64 * j];
+    __syncthreads();
+    // step C: radix-4 over the registers (stride 64), twiddle W256^{L m1}
+    dft4<1>(v[0], v[1], v[2], v[3]);
+    twiddle4<false>(v, tw.t256);
+    lanes64_fwd(v, xch + wave * kTileQ, lane, tw);
+}
+
+__device__ __forceinline__ void fft1024q_inv(cd v[4], cd *xch, int lane, int wave, const TwQ &tw)
+{
+    lanes64_inv(v, xch + wave * kTileQ, lane, tw);
+    twiddle4<true>(v, tw.t256);
+    dft4<-1>(v[0], v[1], v[2], v[3]);
+    // all waves finished their tiles before the exchange overwrites them
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; j++) xch[wave * 256 + lane + 64 * j] = v[j];
+    __syncthreads();
+    const int n0 = lane + 64 * wave;
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = xch[k * 256 + n0];
+    __syncthreads();
+    {
+        cd g = tw.beta;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = cmulc(v[k], g);
+            if (k < 3) g = cmul(g, tw.rho);
+        }
+    }
+    dft4<-1>(v[0], v[1], v[2], v[3]);
+    v[1] = cmulc(v[1], cq(1));
+    v[2] = cmulc(v[2], cq(2));
+    v[3] = cmulc(v[3], cq(3));
+}
+
+// natural frequency of (wave, lane, register q) in the output layout
+__host__ __device__ constexpr int q_freq(int wave, int l, int q)
+{
+    return 4 * ((l >> 4) + 4 * ((l >> 2) & 3) + 16 * (l & 3) + 64 * q) + wave;
+}
+
+// trunc(x) mod 2^64 (execute_direct_torus64's truncation; |x| < 1 gives 0),
+// exact for |x| < 2^85: t = trunc(x) = H 2^32 + L, L in [0, 2^32), H mod 2^32
+__device__ __forceinline__ uint64_t trunc_torus64q(double x)
+{
+    const double t = trunc(x);
+    const double H = floor(t * 0x1p-32);
+    const double L = __fma_rn(-H, 0x1p32, t);
+    const double Hm = __fma_rn(-floor(H * 0x1p-32), 0x1p32, H);
+    return (uint64_t(uint32_t(Hm)) << 32) | uint64_t(uint32_t(L));
+}
+
+__device__ __forceinline__ uint64_t rot_minus_one2q(const uint64_t *poly, uint32_t a, int n)
+{
+    uint32_t idx = (uint32_t(n) - a) & (2u * N2q - 1);
+    uint64_t v = poly[idx & (N2q - 1)];
+    if (idx & N2q) v = 0ull - v;
+    return v - poly[n];
+}
+
+__device__ __forceinline__ double digit2q(uint64_t x, int d)
+{
+    const int sh = 64 - Lvl2::Bgbit * (d + 1);
+    return double(int64_t((x + kDecOff2q) >> sh & ((1ull << Lvl2::Bgbit) - 1)) -
+                  (int64_t(1) << (Lvl2::Bgbit - 1)));
+}
+
+constexpr int kQRow = 4 * 4 * 64;  // double2 per (i, row, column): [wave][q][lane]
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_blind_rotate2q(const uint32_t *__restrict__ in, int n0,
+                                                         const cd *__restrict__ bk,
+                                                         const uint64_t *__restrict__ tv,
+                                                         int tv_stride, uint64_t *__restrict__ out,
+                                                         int sample_extract, int B)
+{
+    __shared__ uint64_t acc[2 * N2q];
+    __shared__ cd xch[4 * kTileQ];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ct = blockIdx.x;
+    if (ct >= B) return;
+    const uint32_t *c0 = in + size_t(ct) * (n0 + 1);
+    const uint64_t *tvp = tv + size_t(ct) * tv_stride;
+    const TwQ tw = make_twq(lane, wave);
+
+    const uint32_t bbar = 2u * N2q - (c0[n0] >> (32 - 1 - Lvl2::nbit));
+    for (int n = tid; n < N2q; n += 256) {
+        acc[n] = 0;
+        uint32_t idx = (uint32_t(n) - bbar) & (2u * N2q - 1);
+        uint64_t v = tvp[idx & (N2q - 1)];
+        acc[N2q + n] = (idx & N2q) ? 0ull - v : v;
+    }
+    __syncthreads();
+
+    constexpr uint32_t kRound = 1u << (32 - 2 - Lvl2::nbit);
+    for (int i = 0; i < n0; i++) {
+        const uint32_t abar = (c0[i] + kRound) >> (32 - 1 - Lvl2::nbit);
+        if (abar == 0) continue;
+        const cd *bki = bk + size_t(i) * Lvl2::rows * 2 * kQRow + wave * 256 + lane;
+        cd fd0[4], fd1[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) fd0[q] = fd1[q] = cd{0.0, 0.0};
+#pragma unroll 1
+        for (int r = 0; r < Lvl2::rows; r++) {
+            const int c = r / Lvl2::l, d = r - c * Lvl2::l;
+            const uint64_t *poly = acc + c * N2q;
+            cd b0[4], b1[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                b0[q] = bki[(r * 2 + 0) * kQRow + q * 64];
+                b1[q] = bki[(r * 2 + 1) * kQRow + q * 64];
+            }
+            cd v[4];
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                const int n = lane + 64 * wave + 256 * p;
+                v[p] = cd{digit2q(rot_minus_one2q(poly, abar, n), d),
+                          digit2q(rot_minus_one2q(poly, abar, n + 1024), d)};
+            }
+            fft1024q_fwd(v, xch, lane, wave, tw);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                cmac(fd0[q], v[q], b0[q]);
+                cmac(fd1[q], v[q], b1[q]);
+            }
+        }
+        // every rotation read of acc is complete (the transforms synchronised)
+        fft1024q_inv(fd0, xch, lane, wave, tw);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const int n = lane + 64 * wave + 256 * p;
+            acc[n] += trunc_torus64q(fd0[p].x);
+            acc[n + 1024] += trunc_torus64q(fd0[p].y);
+        }
+        fft1024q_inv(fd1, xch, lane, wave, tw);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const int n = lane + 64 * wave + 256 * p;
+            acc[N2q + n] += trunc_torus64q(fd1[p].x);
+            acc[N2q + n + 1024] += trunc_torus64q(fd1[p].y);
+        }
+        __syncthreads();
+    }
+
+    if (sample_extract) {
+        uint64_t *o = out + size_t(ct) * (N2q + 1);
+        for (int n = tid; n < N2q; n += 256) o[n] = n == 0 ? acc[0] : 0ull - acc[N2q - n];
+        if (tid == 0) o[N2q] = acc[N2q];
+    } else {
+        uint64_t *o = out + size_t(ct) * 2 * N2q;
+        for (int n = tid; n < 2 * N2q; n += 256) o[n] = acc[n];
+    }
+}
+
+// integer torus polynomial -> key layout [wave][q][lane], scaled by 1/1024
+__global__ __launch_bounds__(256) void k_poly_to_devfd2q(const uint64_t *__restrict__ poly,
+                                                          cd *__restrict__ fd, int npoly)
+{
+    __shared__ cd xch[4 * kTileQ];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int p = blockIdx.x;
+    if (p >= npoly) return;
+    const uint64_t *a = poly + size_t(p) * N2q;
+    const TwQ tw = make_twq(lane, wave);
+    cd v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int n = lane + 64 * wave + 256 * j;
+        v[j] = cd{double(int64_t(a[n])), double(int64_t(a[n + 1024]))};
+    }
+    fft1024q_fwd(v, xch, lane, wave, tw);
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        fd[size_t(p) * kQRow + wave * 256 + q * 64 + lane] =
+            cd{v[q].x * kKeyScale2, v[q].y * kKeyScale2};
+}
+
+// the same transform, spqlios FD order out (parity hook for the layout)
+__global__ __launch_bounds__(256) void k_ifft2q(const uint64_t *__restrict__ poly,
+                                                 double *__restrict__ fd, int B)
+{
+    __shared__ cd xch[4 * kTileQ];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int p = blockIdx.x;
+    if (p >= B) return;
+    const uint64_t *a = poly + size_t(p) * N2q;
+    const TwQ tw = make_twq(lane, wave);
+    cd v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int n = lane + 64 * wave + 256 * j;
+        v[j] = cd{double(int64_t(a[n])), double(int64_t(a[n + 1024]))};
+    }
+    fft1024q_fwd(v, xch, lane, wave, tw);
+    double *o = fd + size_t(p) * N2q;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        int m = q_freq(wave, lane, q), r = 0;
+        for (int b = 0; b < 10; b++) r |= ((m >> b) & 1) << (9 - b);
+        o[r] = v[q].x;
+        o[r + 1024] = v[q].y;
+    }
+}
+
+}  // namespace
+
+size_t bk2q_device_elems(int n0) { return size_t(n0) * Lvl2::rows * 2 * kQRow; }
+
+hipError_t launch_bk2q_from_int(const uint64_t *bk_int, cd *bk_dev, int n0, hipStream_t s)
+{
+    const int npoly = n0 * Lvl2::rows * 2;
+    hipLaunchKernelGGL(k_poly_to_devfd2q, dim3(npoly), dim3(256), 0, s, bk_int, bk_dev, npoly);
+    return hipGetLastError();
+}
+
+hipError_t launch_blind_rotate2q(const uint32_t *in, int n0, const cd *bk, const uint64_t *tv,
+                                 int tv_stride, uint64_t *out, bool sample_extract, int B,
+                                 hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_blind_rotate2q, dim3(B), dim3(256), 0, s, in, n0, bk, tv, tv_stride, out,
+                       int(sample_extract), B);
+    return hipGetLastError();
+}
+
+hipError_t launch_ifft2q(const uint64_t *poly, double *fd, int B, hipStream_t s)
+{
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ifft2q, dim3(B), dim3(256), 0, s, poly, fd, B);
+    return hipGetLastError();
+}
+
+}  // namespace fhe

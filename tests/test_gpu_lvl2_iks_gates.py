@@ -92,6 +92,13 @@ def test_gpu_pbs_lvl02(lvl2, rng):
     assert abs(dphi) <= 1 << 58  # 2^-6 of the torus
     acc = ctx.blind_rotate_lvl02(c0[:2], tv)
     assert np.array_equal(ctx.sample_extract_lvl2(acc, 0), out[:2])
+    # the same key uploaded in TFHEpp's FD form (BootstrappingKeyFFT<lvl02param>)
+    ctx2 = F.Context(k.params)
+    ctx2.load_bk(2, bkfft2, F.BK_FFT)
+    ph2 = O.tlwe_phase64(ctx2.pbs_lvl02(c0, tv), k.sk.s2)
+    assert np.array_equal(np.round(ph2.view(np.int64) / D60).astype(np.int64) % 16, lut[m])
+    assert np.max(np.abs(O.tdiff64(ph2, ph))) <= 1 << 58
+    ctx2.close()
 
 
 def test_gpu_sample_extract_lvl2_bit_exact(lvl2, rng):
