@@ -109,6 +109,23 @@ def cpu_baseline(threads, seconds_budget=12.0):
     }
 
 
+def operator_context(F):
+    """A context with every key the operator layer uses, on the reference's
+    own parameter set (n0 = 672, alpha0 = 2^-16).  The lvl2 comparison chains
+    need it: with n0 = 636 / alpha0 = 2^-15 the key-switched lvl0 noise sits
+    about 3 sigma from the 6-bit windows' decision boundaries."""
+    p = F.preset(F.PRESET_REF672)
+    sk = F.SecretKeys(p, 20261212)
+    ctx = F.Context(p, 0)
+    ctx.load_bk(1, F.keygen_bk_lvl01(sk, 20261213), F.BK_INT)
+    ctx.load_bk(2, F.keygen_bk_lvl02(sk, 20261205), F.BK_INT)
+    for kind, seed in ((10, 20261206), (20, 20261207), (21, 20261208)):
+        ksk = F.keygen_ksk(sk, kind, seed)
+        ctx.load_ksk(kind, ksk)
+        del ksk
+    return ctx, sk, p
+
+
 def run_queries(F, ctx, sk, p, rows, reps):
     """Encrypted query pipelines, each timed with HIP events on the context
     stream over `reps` runs after one untimed run (which also sizes the
@@ -118,13 +135,8 @@ def run_queries(F, ctx, sk, p, rows, reps):
     import ctypes
 
     L_ = F.lib()
-    ctx.load_bk(2, F.keygen_bk_lvl02(sk, 20261205), F.BK_INT)
-    for kind, seed in ((10, 20261206), (20, 20261207), (21, 20261208)):
-        ksk = F.keygen_ksk(sk, kind, seed)
-        ctx.load_ksk(kind, ksk)
-        del ksk
     rng = np.random.default_rng(13)
-    res = {}
+    res = {"preset": f"n0={p.n0} (reference parameter set)"}
 
     def timed(fn):
         fn()
@@ -350,20 +362,22 @@ def main():
         "decrypt_failures": failures,
         "device": F.device_name(),
     }
-    res["queries"] = (None if args.no_query else
-                      run_queries(F, ctx, sk, p, args.query_rows, args.query_reps))
-    if args.mixed:
-        from cufhedb_amd import mixed
-        if args.no_query:  # the stream needs the operator-layer keys
-            run_queries(F, ctx, sk, p, 256, 1)
-        _, res["mixed_stream"] = mixed.run(ctx, sk, p, mixed.make_plan(2026, args.mixed))
-        print(f"bench: mixed stream: {res['mixed_stream']}", file=sys.stderr, flush=True)
+    ctx.close()
+    res["queries"] = None
+    if not args.no_query or args.mixed:
+        octx, osk, op = operator_context(F)
+        if not args.no_query:
+            res["queries"] = run_queries(F, octx, osk, op, args.query_rows, args.query_reps)
+        if args.mixed:
+            from cufhedb_amd import mixed
+            _, res["mixed_stream"] = mixed.run(octx, osk, op, mixed.make_plan(2026, args.mixed))
+            print(f"bench: mixed stream: {res['mixed_stream']}", file=sys.stderr, flush=True)
+        octx.close()
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(threads)
     else:
         res["cpu_baseline"] = None
-    ctx.close()
     print(json.dumps(res))
 
 

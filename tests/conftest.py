@@ -121,6 +121,15 @@ def full636():
     return full_ctx_for(F.PRESET_CFG636)
 
 
+@pytest.fixture(scope="session")
+def full672():
+    """The reference's own parameter set (n0 = 672, alpha0 = 2^-16).  The
+    lvl2 comparison chains run here: with n0 = 636 / alpha0 = 2^-15 the
+    key-switched lvl0 noise (IKS lvl20) sits about 3 sigma from the 6-bit
+    windows' decision boundary, a per-row failure rate near 1e-4."""
+    return full_ctx_for(F.PRESET_REF672)
+
+
 @pytest.fixture
 def rng():
     return np.random.default_rng(20261202)

@@ -73,8 +73,8 @@ def test_compare_lvl1(full636, rng, bits):
 # pb = 28 on; windows next to a slot boundary (plaintexts 0 and -1) are the
 # first to flip.  The reference's ImExtractMSB29/33 share this limit.
 @pytest.mark.parametrize("bits", [12, 20, 26])
-def test_compare_lvl2(full636, rng, bits):
-    k, ctx = full636
+def test_compare_lvl2(full672, rng, bits):
+    k, ctx = full672
     a, b = operands(rng, bits, 16)
     ca, cb = enc(k, 2, a, bits, 7300 + bits), enc(k, 2, b, bits, 7400 + bits)
     for op in (F.CMP_GE, F.CMP_LT, F.CMP_EQ):
@@ -82,9 +82,9 @@ def test_compare_lvl2(full636, rng, bits):
         assert np.array_equal(dec(k, out, F.LOGIC), OPS[op](a, b).astype(np.int64)), (bits, op)
 
 
-def test_compare_lvl2_public_threshold_broadcast(full636, rng):
+def test_compare_lvl2_public_threshold_broadcast(full672, rng):
     # the 20-bit range query x <= t with one encrypted threshold shared by all rows
-    k, ctx = full636
+    k, ctx = full672
     bits = 20
     x = rng.integers(0, 1 << bits, 40)
     t = int(rng.integers(0, 1 << bits))
@@ -97,9 +97,9 @@ def test_compare_lvl2_public_threshold_broadcast(full636, rng):
 
 
 @pytest.mark.parametrize("level,pb", [(1, 5), (1, 7), (1, 10), (2, 6), (2, 14), (2, 24), (2, 27)])
-def test_hom_msb(full636, rng, level, pb):
+def test_hom_msb(full636, full672, rng, level, pb):
     # signed pb-bit plaintexts in the top pb bits; HomMSB = sign bit
-    k, ctx = full636
+    k, ctx = full636 if level == 1 else full672
     w = 32 if level == 1 else 64
     v = rng.integers(0, 1 << pb, 16, dtype=np.uint64)
     v[:4] = [0, (1 << (pb - 1)) - 1, 1 << (pb - 1), (1 << pb) - 1]

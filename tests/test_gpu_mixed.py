@@ -9,8 +9,8 @@ import cufhedb_amd as F
 from cufhedb_amd import mixed
 
 @pytest.mark.gpu
-def test_mixed_stream_small_plan(full636):
-    k, ctx = full636
+def test_mixed_stream_small_plan(full672):
+    k, ctx = full672
     plan = mixed.make_plan(seed=555, nbatches=10, sigma=0.8, frac_lvl1=0.6, max_size=2048)
     assert {kind for kind, _ in plan} <= {"pbs1", "cmp2"}
     records, summary = mixed.run(ctx, k.sk, k.params, plan, seed=556)

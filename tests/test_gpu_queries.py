@@ -22,8 +22,8 @@ def enc1(k, v, bits, seed):
     return F.tlwe_encrypt_u32(seed, 0, k.sk.s1, m, k.params.alpha1)
 
 
-def test_range_filter_lvl2(full636, rng):
-    k, ctx = full636
+def test_range_filter_lvl2(full672, rng):
+    k, ctx = full672
     rows, bits = 1024, 20
     x = rng.integers(0, 1 << bits, rows)
     t = int(rng.integers(1, (1 << bits) - 1))
