@@ -57,6 +57,61 @@ __global__ __launch_bounds__(256) void k_iks(const TD *__restrict__ in,
     }
 }
 
+// Tiled form for small bases (lvl21: basebit 3, 7 key rows per digit
+// position): a workgroup switches C ciphertexts on one 64-column slice of the
+// output, so the 7 candidate rows of a position are read by the C
+// ciphertexts from L1/L2 instead of once per ciphertext from HBM.  Each wave
+// takes C/4 ciphertexts (wave-uniform, so their a_i are scalar loads); each
+// lane owns one output column.
+template <class TD, int DN, int T, int BB, int C>
+__global__ __launch_bounds__(256) void k_iks_tiled(const TD *__restrict__ in,
+                                                   const uint32_t *__restrict__ ksk,
+                                                   uint32_t *__restrict__ out, int tn, int B)
+{
+    constexpr int W = int(sizeof(TD) * 8);
+    constexpr uint32_t mask = (1u << BB) - 1;
+    constexpr size_t nd = (size_t(1) << BB) - 1;
+    constexpr int PER = C / 4;
+    const int col = blockIdx.y * 64 + (threadIdx.x & 63);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int b0 = blockIdx.x * C + wv * PER;
+    const size_t row = size_t(tn) + 1;
+    const bool colok = col <= tn;
+    const TD prec = TD(1) << (W - (1 + BB * T));
+    uint32_t acc[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int b = b0 + k;
+        uint32_t body = 0;
+        if (b < B && col == tn) {
+            const TD v = in[size_t(b) * (DN + 1) + DN];
+            body = W == 32 ? uint32_t(v) : uint32_t((v + (TD(1) << (W - 33))) >> (W - 32));
+        }
+        acc[k] = body;
+    }
+    const uint32_t *kc = ksk + (colok ? col : 0);
+    for (int i = 0; i < DN; i++) {
+        TD abar[PER];
+#pragma unroll
+        for (int k = 0; k < PER; k++)
+            abar[k] = (b0 + k < B) ? TD(in[size_t(b0 + k) * (DN + 1) + i] + prec) : TD(0);
+#pragma unroll
+        for (int j = 0; j < T; j++) {
+            const uint32_t *kr = kc + (size_t(i) * T + j) * nd * row;
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                const uint32_t d = uint32_t(abar[k] >> (W - (j + 1) * BB)) & mask;
+                if (d != 0 && colok) acc[k] -= kr[(d - 1) * row];
+            }
+        }
+    }
+    if (colok) {
+#pragma unroll
+        for (int k = 0; k < PER; k++)
+            if (b0 + k < B) out[size_t(b0 + k) * row + col] = acc[k];
+    }
+}
+
 __global__ void k_tlwe1_lincomb(const uint32_t *__restrict__ a, const uint32_t *__restrict__ b,
                                 int sa, int sb, uint32_t offset, uint32_t *__restrict__ out,
                                 int B)
@@ -233,10 +288,13 @@ hipError_t launch_iks(int kind, const void *in, const uint32_t *ksk, uint32_t *o
         hipLaunchKernelGGL((k_iks<uint64_t, Lvl2::N, KS20::t, KS20::basebit>), dim3(B), dim3(256),
                            0, s, static_cast<const uint64_t *>(in), ksk, out, tn, B);
         break;
-    case 21:
-        hipLaunchKernelGGL((k_iks<uint64_t, Lvl2::N, KS21::t, KS21::basebit>), dim3(B), dim3(256),
-                           0, s, static_cast<const uint64_t *>(in), ksk, out, tn, B);
+    case 21: {
+        constexpr int C = 32;
+        const dim3 grid(unsigned((B + C - 1) / C), unsigned((tn + 1 + 63) / 64));
+        hipLaunchKernelGGL((k_iks_tiled<uint64_t, Lvl2::N, KS21::t, KS21::basebit, C>), grid,
+                           dim3(256), 0, s, static_cast<const uint64_t *>(in), ksk, out, tn, B);
         break;
+    }
     default:
         return hipErrorInvalidValue;
     }

@@ -143,14 +143,22 @@ def test_gpu_iks_from_lvl2_bit_exact(kind, keys636, rng):
     ksk = F.keygen_ksk(k.sk, kind, 636700 + kind)
     ctx = F.Context(k.params)
     ctx.load_ksk(kind, ksk)
-    B = 16
+    # 45 rows: a full 32-ciphertext tile of the lvl21 kernel plus a ragged one
+    B = 45
     m = (rng.integers(0, 8, B).astype(np.uint64) << np.uint64(60))
     c2 = F.tlwe_encrypt_u64(636800, 0, k.sk.s2, m, 2.0 ** -52)
     got = ctx.iks(kind, c2)
     tn, t, bb = (k.n0, 2, 10) if kind == 20 else (1024, 10, 3)
     key = k.sk.s0 if kind == 20 else k.sk.s1
-    for b in range(4):
-        assert np.array_equal(got[b], O.iks64(c2[b], ksk, tn, t, bb))
+    for b in range(B):
+        assert np.array_equal(got[b], O.iks64(c2[b], ksk, tn, t, bb)), b
+    # all-zero and all-ones masks: every digit 0 / every digit at its maximum
+    edge = np.zeros((3, c2.shape[1]), np.uint64)
+    edge[1, :] = np.uint64(2**64 - 1)
+    edge[2, :-1] = np.uint64(2**63)
+    got_e = ctx.iks(kind, edge)
+    for b in range(3):
+        assert np.array_equal(got_e[b], O.iks64(edge[b], ksk, tn, t, bb)), b
     e = O.tdiff32(F.tlwe_phase_u32(got, key), (m >> np.uint64(32)).astype(np.uint32))
     assert np.max(np.abs(e)) < 1 << 26
     ctx.close()
