@@ -120,9 +120,7 @@ def operator_context(F):
     ctx.load_bk(1, F.keygen_bk_lvl01(sk, 20261213), F.BK_INT)
     ctx.load_bk(2, F.keygen_bk_lvl02(sk, 20261205), F.BK_INT)
     for kind, seed in ((10, 20261206), (20, 20261207), (21, 20261208)):
-        ksk = F.keygen_ksk(sk, kind, seed)
-        ctx.load_ksk(kind, ksk)
-        del ksk
+        ctx.keygen_ksk(kind, sk, seed)  # on the device; same key as keygen_ksk
     return ctx, sk, p
 
 

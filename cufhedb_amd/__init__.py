@@ -96,6 +96,7 @@ _SIGS = {
     "fhe_ctx_stream": (_vp, [_vp]),
     "fhe_load_bk": (_i, [_vp, _i, _vp, _i]),
     "fhe_load_ksk": (_i, [_vp, _i, _vp]),
+    "fhe_keygen_ksk_dev": (_i, [_vp, _i, _u64, _vp, _vp, _vp]),
     "fhe_set_pbs_variant": (_i, [_vp, _i]),
     "fhe_pbs_lvl01_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp]),
     "fhe_blind_rotate_lvl01_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp]),
@@ -121,6 +122,16 @@ _SIGS = {
     "fhe_fwd_digits_lvl1_mfma_dev": (_i, [_vp, _vp, _vp, _i, _vp]),
     "fhe_compare_dev": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _u32, _i, _i, _vp]),
     "fhe_hom_not_dev": (_i, [_vp, _i, _vp, _vp, _i, _vp]),
+    "fhe_keyfile_params_of": (_i, [_vp, _vp]),
+    "fhe_keyfile_payload_bytes": (_u64, [_i, _u32]),
+    "fhe_keyfile_info": (_i, [ctypes.c_char_p, _i, _vp, _vp]),
+    "fhe_keyfile_read": (_i, [ctypes.c_char_p, _i, _vp, _u64]),
+    "fhe_keyfile_read_secret": (_i, [ctypes.c_char_p, _vp, _u32, _vp, _vp, _vp]),
+    "fhe_keyfile_write": (_i, [ctypes.c_char_p, _i, _vp, _vp]),
+    "fhe_keyfile_write_secret": (_i, [ctypes.c_char_p, _vp, _vp, _vp, _vp]),
+    "fhe_keyfile_load": (_i, [_vp, ctypes.c_char_p, _i]),
+    "fhe_fastr_write": (_i, [ctypes.c_char_p, _u32, _vp, _vp, _vp]),
+    "fhe_fastr_load": (_i, [_vp, ctypes.c_char_p, _vp]),
     "fhe_pbs_lvl01": (_i, [_vp, _vp, _vp, _i, _vp, _i]),
     "fhe_iks": (_i, [_vp, _i, _vp, _vp, _i]),
     "fhe_malloc": (_i, [_vp, _sz]),
@@ -332,6 +343,15 @@ class Context:
     def load_ksk(self, kind, ksk):
         ksk = np.ascontiguousarray(ksk, np.uint32)
         check(lib().fhe_load_ksk(self.h, kind, _ptr(ksk)))
+
+    def keygen_ksk(self, kind, sk, seed, out_dev=None):
+        """Key-switching key generated on the device, bit-identical to
+        keygen_ksk(sk, kind, seed); into the context's key slot, or into the
+        DeviceBuffer out_dev."""
+        sdom = sk.s1 if kind == 10 else sk.s2
+        stgt = sk.s1 if kind == 21 else sk.s0
+        check(lib().fhe_keygen_ksk_dev(self.h, kind, seed, _ptr(sdom), _ptr(stgt),
+                                       out_dev.ptr if out_dev is not None else None))
 
     def set_pbs_variant(self, variant):
         check(lib().fhe_set_pbs_variant(self.h, variant))

@@ -7,20 +7,29 @@
 #include <cmath>
 #include <cstdint>
 
+// Philox is also evaluated by device code (keygen_dev.hip); the host and
+// device versions are the same integer arithmetic.
+#if defined(__HIPCC__)
+#define FHE_HD __host__ __device__
+#else
+#define FHE_HD
+#endif
+
 namespace fhe {
 
 struct Philox {
     uint32_t k0, k1;
-    explicit Philox(uint64_t seed) : k0(uint32_t(seed)), k1(uint32_t(seed >> 32)) {}
+    FHE_HD explicit Philox(uint64_t seed) : k0(uint32_t(seed)), k1(uint32_t(seed >> 32)) {}
 
-    static inline void mulhilo(uint32_t a, uint32_t b, uint32_t &hi, uint32_t &lo)
+    FHE_HD static inline void mulhilo(uint32_t a, uint32_t b, uint32_t &hi, uint32_t &lo)
     {
         uint64_t p = uint64_t(a) * b;
         hi = uint32_t(p >> 32);
         lo = uint32_t(p);
     }
     // 4 words for counter (c0..c3)
-    inline void block(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t out[4]) const
+    FHE_HD inline void block(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                             uint32_t out[4]) const
     {
         uint32_t key0 = k0, key1 = k1;
         for (int r = 0; r < 10; r++) {
@@ -62,6 +71,13 @@ class Stream {
         return buf_[pos_++];
     }
     inline uint64_t u64() { uint64_t lo = u32(); return lo | (uint64_t(u32()) << 32); }
+    // position the stream at word w of the record (the next u32() returns it)
+    inline void seek(uint64_t w)
+    {
+        ctr_ = uint32_t(w >> 2);
+        pos_ = 4;
+        for (uint64_t i = 0; i < (w & 3); i++) (void)u32();
+    }
     // uniform double in (0, 1], 53-bit
     inline double unit_open0()
     {

@@ -121,6 +121,15 @@ int fhe_load_bk(fhe_ctx *ctx, int level /* 1 or 2 */, const void *bk_host, int f
 /* Key-switching key upload (host TFHEpp layout, kind 10/20/21).  The
  * reference keeps these on the host and switches on the CPU. */
 int fhe_load_ksk(fhe_ctx *ctx, int kind, const uint32_t *ksk_host);
+/* Key-switching key generated on the device (ikskgen,
+ * thirdparty/TFHEpp/src/cloudkey.cpp:54-70), bit-identical to fhe_keygen_ksk
+ * for the same seed and keys: the device draws the masks and <a, s>, the
+ * host only the noise word of each record.  ksk_dev = NULL writes the
+ * context's key slot (as fhe_load_ksk would); otherwise a device buffer of
+ * fhe_ksk_words(kind) words.  s_dom / s_tgt are host keys as for
+ * fhe_keygen_ksk.  Synchronous. */
+int fhe_keygen_ksk_dev(fhe_ctx *ctx, int kind, uint64_t seed, const void *s_dom,
+                       const uint32_t *s_tgt, uint32_t *ksk_dev);
 
 /* PBS kernel variant for the lvl1 blind rotation. */
 enum fhe_pbs_variant {
@@ -261,6 +270,69 @@ int fhe_compare_dev(fhe_ctx *ctx, int level, int op, const void *a_dev, const vo
  * every word of B TLWEs of level 0, 1 or 2 (out may equal in). */
 int fhe_hom_not_dev(fhe_ctx *ctx, int level, const void *in_dev, void *out_dev, int B,
                     void *stream);
+
+/* ---- key material files ------------------------------------------------- */
+/* The reference's key files: TFHEpp cereal PortableBinary archives written by
+ * HEDB::generate_*_key and read by HEDB::load_*_key
+ * (src/HEDB/utils/serialize.cpp:7-320; file names ./../evalkey/<name>.key),
+ * and the raw fastR dumps (test/HE3_Query/fastR.h:6-74).  Layouts:
+ *   secret.key      SecretKey::serialize (TFHEpp/include/key.hpp:29-35):
+ *                   flag byte, lvl0 key [n0] u32, lvl1 key [1024] u32,
+ *                   lvl2 key [2048] u64, lweParams
+ *   <member>.key    EvalKey::serialize_<member> (TFHEpp/include/cloudkey.hpp:
+ *                   164-228): flag byte, lweParams, presence byte, the array
+ * The flag byte is cereal's endianness mark (1 = little-endian); big-endian
+ * files are byte-swapped on read, files are written little-endian. */
+enum fhe_keyfile {
+    FHE_KEYFILE_SECRET = 0,     /* secret.key                                  */
+    FHE_KEYFILE_BKLVL01 = 1,    /* bklvl01.key     [n0][6][2][1024] uint32_t   */
+    FHE_KEYFILE_BKLVL02 = 2,    /* bklvl02.key     [n0][8][2][2048] uint64_t   */
+    FHE_KEYFILE_BKFFTLVL01 = 3, /* bkfftlvl01.key  [n0][6][2][1024] double     */
+    FHE_KEYFILE_BKFFTLVL02 = 4, /* bkfftlvl02.key  [n0][8][2][2048] double     */
+    FHE_KEYFILE_IKSKLVL10 = 5,  /* iksklvl10.key   [1024][2][1023][n0+1] u32   */
+    FHE_KEYFILE_IKSKLVL20 = 6,  /* iksklvl20.key   [2048][2][1023][n0+1] u32   */
+    FHE_KEYFILE_IKSKLVL21 = 7   /* iksklvl21.key   [2048][10][7][1025] u32     */
+};
+/* lweParams as stored (TFHEpp/include/lweParams.hpp:181-190), field order. */
+typedef struct fhe_keyfile_params {
+    uint32_t n0;  double alpha0;  uint32_t approx_bit0;
+    uint32_t nbit1, n1, l1, bgbit1;  double alpha1;  uint32_t approx_bit1;
+    uint32_t nbit2, n2, l2, bgbit2;  double alpha2;  uint32_t approx_bit2;
+    uint32_t t10, basebit10;  double alpha10;
+    uint32_t t20, basebit20;  double alpha20;
+    uint32_t t21, basebit21;  double alpha21;
+    uint32_t t22, basebit22;  double alpha22;
+} fhe_keyfile_params;
+/* The parameter block this library writes for p (128bit.hpp values, p's n0
+ * and noise). */
+int fhe_keyfile_params_of(const fhe_params *p, fhe_keyfile_params *out);
+/* Bytes of a file's key array (secret: all three keys). 0 for a bad kind. */
+uint64_t fhe_keyfile_payload_bytes(int kind, uint32_t n0);
+/* Parse and check a file's header and size; either output may be NULL. */
+int fhe_keyfile_info(const char *path, int kind, fhe_keyfile_params *params,
+                     uint64_t *payload_bytes);
+/* HEDB::load_<member>_key into host memory (kinds 1-7). */
+int fhe_keyfile_read(const char *path, int kind, void *dst, uint64_t dst_bytes);
+/* HEDB::load_secret_key (serialize.cpp:17-24); params may be NULL. */
+int fhe_keyfile_read_secret(const char *path, uint32_t *s0, uint32_t n0_cap, uint32_t *s1,
+                            uint64_t *s2, fhe_keyfile_params *params);
+/* HEDB::generate_<member>_key's archive step (serialize.cpp:26-313) for a
+ * key the caller holds in TFHEpp layout. */
+int fhe_keyfile_write(const char *path, int kind, const fhe_params *p, const void *payload);
+/* HEDB::generate_secret_key's archive step (serialize.cpp:7-15). */
+int fhe_keyfile_write_secret(const char *path, const fhe_params *p, const uint32_t *s0,
+                             const uint32_t *s1, const uint64_t *s2);
+/* HEDB::load_<member>_key straight into a context: bootstrapping keys go
+ * through fhe_load_bk, key-switching keys stream from the file to HBM in
+ * 64 MiB pinned chunks.  The file's n0 must equal the context's. */
+int fhe_keyfile_load(fhe_ctx *ctx, const char *path, int kind);
+/* fastR.h writeToFile / readFromFile (test/HE3_Query/fastR.h:6-74): lvl1
+ * key, BootstrappingKeyFFT<lvl01param>, KeySwitchingKey<lvl10param>, raw.
+ * fhe_fastr_load loads the two keys into ctx (n0 = ctx's) and returns the
+ * lvl1 secret key in s1_out (1024 words) when s1_out is not NULL. */
+int fhe_fastr_write(const char *path, uint32_t n0, const uint32_t *s1, const double *bkfft,
+                    const uint32_t *iksk10);
+int fhe_fastr_load(fhe_ctx *ctx, const char *path, uint32_t *s1_out);
 
 /* ---- host-pointer convenience forms (synchronous, like the *_st calls) -- */
 int fhe_pbs_lvl01(fhe_ctx *ctx, const uint32_t *in, const uint32_t *tv, int tv_stride,

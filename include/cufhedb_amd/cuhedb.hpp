@@ -26,6 +26,9 @@
 //   * Batched forms (the *_batch members) take B ciphertexts per call; that
 //     is the fast path.
 //   * Errors throw cufhedb_amd::Error (the reference ignores CUDA errors).
+//   HEDB:: key files                  src/HEDB/utils/serialize.h:6-61
+//     load_secret_key, load_bk*/bkfft*/iksk*_key (from ./../evalkey/ by
+//     default, as there) -- the evaluation keys go straight into a Runtime
 #pragma once
 
 #include <array>
@@ -430,3 +433,44 @@ inline void log_rescale(cufhedb_amd::Runtime &rt, TLWELvl1 &res, const TLWELvl1 
 }
 
 }  // namespace cuHEDB
+
+// HEDB key files (src/HEDB/utils/serialize.cpp:7-320).  The reference's
+// load_*_key fill a TFHEpp::EvalKey from ./../evalkey/<name>.key; these
+// load the same files into a Runtime's device keys.  The generate_*_key
+// archive step is fhe_keyfile_write / fhe_keyfile_write_secret.
+namespace HEDB {
+
+struct SecretKey {
+    std::vector<uint32_t> lvl0;
+    std::array<uint32_t, 1024> lvl1;
+    std::array<uint64_t, 2048> lvl2;
+    fhe_keyfile_params params;
+};
+
+inline void load_secret_key(SecretKey &sk, const std::string &dir = "./../evalkey")
+{
+    const std::string path = dir + "/secret.key";
+    fhe_keyfile_params q;
+    cufhedb_amd::check(fhe_keyfile_info(path.c_str(), FHE_KEYFILE_SECRET, &q, nullptr),
+                       "load_secret_key");
+    sk.lvl0.assign(q.n0, 0);
+    cufhedb_amd::check(fhe_keyfile_read_secret(path.c_str(), sk.lvl0.data(), q.n0, sk.lvl1.data(),
+                                               sk.lvl2.data(), &sk.params),
+                       "load_secret_key");
+}
+
+#define CUFHEDB_AMD_LOADKEY(NAME, KIND, FILE)                                              \
+    inline void NAME(cufhedb_amd::Runtime &rt, const std::string &dir = "./../evalkey")    \
+    {                                                                                     \
+        cufhedb_amd::check(fhe_keyfile_load(rt.ctx(), (dir + "/" FILE).c_str(), KIND), #NAME); \
+    }
+CUFHEDB_AMD_LOADKEY(load_bklvl01_key, FHE_KEYFILE_BKLVL01, "bklvl01.key")
+CUFHEDB_AMD_LOADKEY(load_bklvl02_key, FHE_KEYFILE_BKLVL02, "bklvl02.key")
+CUFHEDB_AMD_LOADKEY(load_bkfftlvl01_key, FHE_KEYFILE_BKFFTLVL01, "bkfftlvl01.key")
+CUFHEDB_AMD_LOADKEY(load_bkfftlvl02_key, FHE_KEYFILE_BKFFTLVL02, "bkfftlvl02.key")
+CUFHEDB_AMD_LOADKEY(load_iksklvl10_key, FHE_KEYFILE_IKSKLVL10, "iksklvl10.key")
+CUFHEDB_AMD_LOADKEY(load_iksklvl20_key, FHE_KEYFILE_IKSKLVL20, "iksklvl20.key")
+CUFHEDB_AMD_LOADKEY(load_iksklvl21_key, FHE_KEYFILE_IKSKLVL21, "iksklvl21.key")
+#undef CUFHEDB_AMD_LOADKEY
+
+}  // namespace HEDB

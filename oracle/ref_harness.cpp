@@ -10,6 +10,7 @@
 // TFHEpp is compiled with its default parameter set params/128bit.hpp, i.e.
 // lvl0 n = 672; the n = 636 set is checked through the oracle only.
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <tfhe++.hpp>
 
@@ -219,6 +220,111 @@ void ref_ide_gate_lvl1(uint32_t *out, const uint32_t *in, uint32_t scale_bits,
     TLWE<lvl1param> r;
     IdeGateBootstrapping(r, as<TLWE<lvl1param>>(in), scale_bits, k.ek);
     std::memcpy(out, r.data(), sizeof(r));
+}
+
+}  // extern "C"
+
+// ---- key files ------------------------------------------------------------
+// The reference's own archive code: HEDB::generate_*_key / load_*_key
+// (src/HEDB/utils/serialize.cpp:7-320) run cereal's PortableBinary archive
+// over SecretKey::serialize and EvalKey::serialize_<member>.  `kind` numbers
+// the member files as enum fhe_keyfile in include/cufhedb_amd.h does
+// (1 bklvl01, 2 bklvl02, 3 bkfftlvl01, 4 bkfftlvl02, 5 iksklvl10,
+// 6 iksklvl20, 7 iksklvl21).
+namespace {
+template <class M>
+int member_io(std::unique_ptr<M> &slot, void *data, bool save, bool be, EvalKey &ek,
+              void (EvalKey::*ser_out)(cereal::PortableBinaryOutputArchive &),
+              void (EvalKey::*ser_in)(cereal::PortableBinaryInputArchive &), const char *path)
+{
+    if (save) {
+        slot.reset(reinterpret_cast<M *>(data));
+        {
+            std::ofstream ofs{path, std::ios::binary};
+            cereal::PortableBinaryOutputArchive ar(
+                ofs, be ? cereal::PortableBinaryOutputArchive::Options::BigEndian()
+                        : cereal::PortableBinaryOutputArchive::Options::LittleEndian());
+            (ek.*ser_out)(ar);
+        }
+        slot.release();
+        return 0;
+    }
+    {
+        std::ifstream ifs{path, std::ios::binary};
+        cereal::PortableBinaryInputArchive ar(ifs);
+        (ek.*ser_in)(ar);
+    }
+    if (!slot) return 1;
+    std::memcpy(data, slot.get(), sizeof(M));
+    return 0;
+}
+
+int keyfile_io(const char *path, int kind, void *data, bool save, bool be)
+{
+    EvalKey ek;
+    using O = cereal::PortableBinaryOutputArchive;
+    using I = cereal::PortableBinaryInputArchive;
+    switch (kind) {
+    case 1: return member_io(ek.bklvl01, data, save, be, ek, &EvalKey::serialize_bklvl01<O>,
+                             &EvalKey::serialize_bklvl01<I>, path);
+    case 2: return member_io(ek.bklvl02, data, save, be, ek, &EvalKey::serialize_bklvl02<O>,
+                             &EvalKey::serialize_bklvl02<I>, path);
+    case 3: return member_io(ek.bkfftlvl01, data, save, be, ek, &EvalKey::serialize_bkfftlvl01<O>,
+                             &EvalKey::serialize_bkfftlvl01<I>, path);
+    case 4: return member_io(ek.bkfftlvl02, data, save, be, ek, &EvalKey::serialize_bkfftlvl02<O>,
+                             &EvalKey::serialize_bkfftlvl02<I>, path);
+    case 5: return member_io(ek.iksklvl10, data, save, be, ek, &EvalKey::serialize_iksklvl10<O>,
+                             &EvalKey::serialize_iksklvl10<I>, path);
+    case 6: return member_io(ek.iksklvl20, data, save, be, ek, &EvalKey::serialize_iksklvl20<O>,
+                             &EvalKey::serialize_iksklvl20<I>, path);
+    case 7: return member_io(ek.iksklvl21, data, save, be, ek, &EvalKey::serialize_iksklvl21<O>,
+                             &EvalKey::serialize_iksklvl21<I>, path);
+    default: return 2;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+// big_endian != 0 writes the archive with cereal's big-endian option
+int ref_keyfile_write(const char *path, int kind, const void *data, int big_endian)
+{
+    return keyfile_io(path, kind, const_cast<void *>(data), true, big_endian != 0);
+}
+
+int ref_keyfile_read(const char *path, int kind, void *out)
+{
+    return keyfile_io(path, kind, out, false, false);
+}
+
+int ref_secret_write(const char *path, const uint32_t *s0, const uint32_t *s1, const uint64_t *s2,
+                     int big_endian)
+{
+    auto sk = std::make_unique<SecretKey>();
+    std::memcpy(sk->key.lvl0.data(), s0, sizeof(sk->key.lvl0));
+    std::memcpy(sk->key.lvl1.data(), s1, sizeof(sk->key.lvl1));
+    std::memcpy(sk->key.lvl2.data(), s2, sizeof(sk->key.lvl2));
+    std::ofstream ofs{path, std::ios::binary};
+    cereal::PortableBinaryOutputArchive ar(
+        ofs, big_endian ? cereal::PortableBinaryOutputArchive::Options::BigEndian()
+                        : cereal::PortableBinaryOutputArchive::Options::LittleEndian());
+    sk->serialize(ar);
+    return 0;
+}
+
+int ref_secret_read(const char *path, uint32_t *s0, uint32_t *s1, uint64_t *s2, uint32_t *n0)
+{
+    auto sk = std::make_unique<SecretKey>();
+    {
+        std::ifstream ifs{path, std::ios::binary};
+        cereal::PortableBinaryInputArchive ar(ifs);
+        sk->serialize(ar);
+    }
+    std::memcpy(s0, sk->key.lvl0.data(), sizeof(sk->key.lvl0));
+    std::memcpy(s1, sk->key.lvl1.data(), sizeof(sk->key.lvl1));
+    std::memcpy(s2, sk->key.lvl2.data(), sizeof(sk->key.lvl2));
+    *n0 = sk->params.lvl0.n;
+    return 0;
 }
 
 }  // extern "C"

@@ -101,7 +101,8 @@ _FULL = {}
 
 def full_ctx_for(which):
     """Context holding every key the operator layer uses: BK lvl01 and lvl02,
-    KSK 10, 20 and 21 (host copies of the KSKs are dropped after upload)."""
+    KSK 10, 20 and 21 (20 and 21 generated on the device, bit-identical to the
+    host generator's keys for the same seeds)."""
     if which not in _FULL:
         k = keys_for(which)
         ctx = F.Context(k.params)
@@ -109,9 +110,7 @@ def full_ctx_for(which):
         ctx.load_bk(2, F.keygen_bk_lvl02(k.sk, k.seed + 3), F.BK_INT)
         ctx.load_ksk(10, k.ksk10)
         for kind, off in ((20, 4), (21, 5)):
-            ksk = F.keygen_ksk(k.sk, kind, k.seed + off)
-            ctx.load_ksk(kind, ksk)
-            del ksk
+            ctx.keygen_ksk(kind, k.sk, k.seed + off)
         _FULL[which] = ctx
     return keys_for(which), _FULL[which]
 

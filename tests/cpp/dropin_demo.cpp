@@ -1,8 +1,13 @@
 // Exercises the C++ drop-in (include/cufhedb_amd/cuhedb.hpp) the way a
 // cuFHEDB caller would: CuFFT_Torus<P>::LoadBK / *_st calls and the cuHEDB::
 // operators.  Exit status 0 = every check passed.  Needs a GPU.
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
 #include <cstdio>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "cufhedb_amd/cuhedb.hpp"
@@ -29,12 +34,27 @@ int main()
     std::vector<uint32_t> ksk(fhe_ksk_words(&p, 10));
     cufhedb_amd::check(fhe_keygen_ksk(&p, 10, 9, s1.data(), s0.data(), ksk.data()), "ksk");
 
+    // the key-switching key travels through the reference's key-file format:
+    // written as HEDB::generate_iksklvl10_key would, loaded by
+    // HEDB::load_iksklvl10_key; the secret key likewise
+    const std::string dir = "/tmp/dropin_evalkey_" + std::to_string(getpid());
+    mkdir(dir.c_str(), 0700);
+    cufhedb_amd::check(fhe_keyfile_write((dir + "/iksklvl10.key").c_str(), FHE_KEYFILE_IKSKLVL10,
+                                         &p, ksk.data()), "write iksk");
+    cufhedb_amd::check(fhe_keyfile_write_secret((dir + "/secret.key").c_str(), &p, s0.data(),
+                                                s1.data(), s2.data()), "write secret");
+    HEDB::SecretKey sk;
+    HEDB::load_secret_key(sk, dir);
+    int bad = sk.lvl0 != s0 || !std::equal(s1.begin(), s1.end(), sk.lvl1.begin());
+
     cufhedb_amd::Runtime rt(p, 0);
-    rt.load_ksk(10, ksk.data());
+    HEDB::load_iksklvl10_key(rt, dir);
+    std::remove((dir + "/iksklvl10.key").c_str());
+    std::remove((dir + "/secret.key").c_str());
+    rmdir(dir.c_str());
     cufft::CuFFT_Torus<lvl1param> cufftlvl1(rt);
     cufftlvl1.LoadBKInt(bk.data());
 
-    int bad = 0;
     std::mt19937 gen(3);
     // GateBootstrappingTLWE2TLWEFFT_st with constant test vector u: output
     // +u for phase in [0, 1/2), -u otherwise
