@@ -49,9 +49,9 @@ constexpr uint32_t kDigitFlip1 = (1u << 31) | (1u << 25) | (1u << 19);
 __device__ __forceinline__ uint32_t prep1(uint32_t x) { return (x + kDecOff1) ^ kDigitFlip1; }
 __device__ __forceinline__ double digit1p(uint32_t u, int d)
 {
-    // field [sh, sh + 6) moved to the top, then arithmetic shift down
-    const int up = Lvl1::Bgbit * d;  // = 32 - 6 - sh
-    return double(int32_t(u << up) >> (32 - Lvl1::Bgbit));
+    // signed extract of the field [sh, sh + 6), sh = 32 - 6 (d + 1)
+    const int sh = 32 - Lvl1::Bgbit * (d + 1);
+    return double(int32_t(__builtin_amdgcn_sbfe(int32_t(u), sh, Lvl1::Bgbit)));
 }
 __device__ __forceinline__ double digit1(uint32_t x, int d) { return digit1p(prep1(x), d); }
 

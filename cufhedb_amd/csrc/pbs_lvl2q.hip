@@ -218,11 +218,17 @@ __device__ __forceinline__ uint64_t rot_minus_one2q(const uint64_t *poly, uint32
     return v - poly[n];
 }
 
-__device__ __forceinline__ double digit2q(uint64_t x, int d)
+// Gadget digits of x (DecompositionPolynomial, trgsw.cpp:21-39):
+//   digit_d = ((x + offset) >> (64 - 9(d+1)) & 511) - 256.
+// prep2q adds the offset and flips the top bit of every 9-bit field, so each
+// digit is one signed 9-bit field extract of the prepared word: (f ^ 256)
+// read as a signed 9-bit number equals f - 256.
+constexpr uint64_t kDigitFlip2q = (1ull << 63) | (1ull << 54) | (1ull << 45) | (1ull << 36);
+__device__ __forceinline__ uint64_t prep2q(uint64_t x) { return (x + kDecOff2q) ^ kDigitFlip2q; }
+__device__ __forceinline__ double digit2qp(uint64_t u, int d)
 {
     const int sh = 64 - Lvl2::Bgbit * (d + 1);
-    return double(int64_t((x + kDecOff2q) >> sh & ((1ull << Lvl2::Bgbit) - 1)) -
-                  (int64_t(1) << (Lvl2::Bgbit - 1)));
+    return double(int32_t(__builtin_amdgcn_sbfe(int32_t(uint32_t(u >> sh)), 0, Lvl2::Bgbit)));
 }
 
 constexpr int kQRow = 4 * 4 * 64;  // double2 per (i, row, column): [wave][q][lane]
@@ -274,8 +280,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 #pragma unroll
             for (int p = 0; p < 4; p++) {
                 const int n = lane + 64 * wave + 256 * p;
-                v[p] = cd{digit2q(rot_minus_one2q(poly, abar, n), d),
-                          digit2q(rot_minus_one2q(poly, abar, n + 1024), d)};
+                v[p] = cd{digit2qp(prep2q(rot_minus_one2q(poly, abar, n)), d),
+                          digit2qp(prep2q(rot_minus_one2q(poly, abar, n + 1024)), d)};
             }
             fft1024q_fwd(v, xch, lane, wave, tw);
 #pragma unroll
