@@ -178,7 +178,7 @@ int fhe_load_ksk(fhe_ctx *ctx, int kind, const uint32_t *ksk_host)
     FHE_CHECK_ARG(slot >= 0, "fhe_load_ksk: kind must be 10, 20 or 21 (got %d)", kind);
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     const size_t bytes = fhe_ksk_words(&ctx->p, kind) * sizeof(uint32_t);
-    if (!ctx->ksk[slot]) HIP_TRY(hipMalloc(&ctx->ksk[slot], bytes), "KSK allocation");
+    if (int rc = alloc_ksk(ctx, kind)) return rc;
     HIP_TRY(hipMemcpy(ctx->ksk[slot], ksk_host, bytes, hipMemcpyHostToDevice), "KSK upload");
     return FHE_OK;
 }

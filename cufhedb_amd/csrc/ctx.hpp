@@ -74,6 +74,20 @@ inline int grow(uint32_t *&buf, size_t &cap, size_t words)
     return FHE_OK;
 }
 
+// Device key-switching keys are allocated with kKskPadWords spare words at
+// the end: the whole-row key-switch kernel reads every row in 64-word chunks
+// and may run up to 63 words past the end of the last row (those words are
+// discarded).
+constexpr size_t kKskPadWords = 64;
+inline int alloc_ksk(fhe_ctx *ctx, int kind)
+{
+    const int slot = ksk_slot(kind);
+    if (ctx->ksk[slot]) return FHE_OK;
+    const size_t bytes = (fhe_ksk_words(&ctx->p, kind) + kKskPadWords) * sizeof(uint32_t);
+    HIP_TRY(hipMalloc(&ctx->ksk[slot], bytes), "KSK allocation");
+    return FHE_OK;
+}
+
 inline int ks_target_n(const fhe_ctx *ctx, int kind) { return kind == 21 ? Lvl1::N : int(ctx->p.n0); }
 inline int ks_domain_words(int kind) { return kind == 10 ? Lvl1::N + 1 : Lvl2::N + 1; }
 

@@ -57,6 +57,81 @@ __global__ __launch_bounds__(256) void k_iks(const TD *__restrict__ in,
     }
 }
 
+// Whole-row form for large bases (kinds 10 and 20: basebit 10, 1023
+// candidate rows per digit position, so ciphertexts rarely share a row): one
+// workgroup per ciphertext, wave w takes every fourth digit position and
+// reads each selected row whole (tn + 1 words as NCH = ceil((tn + 1) / 64)
+// coalesced loads, one per lane and 64 columns), two rows at a time, so the
+// memory system sees 2.5 KB contiguous requests with 2 NCH loads in flight
+// per wave; the four waves' partial sums meet in LDS.  The loop body has no
+// branch: a zero digit reads a valid row and discards it, and lanes past the
+// last column read up to 63 words beyond the row (the device key carries
+// kKskPadWords spare words) into sums that are never written.
+template <class TD, int DN, int T, int BB, int NCH>
+__global__ __launch_bounds__(256) void k_iks_rows(const TD *__restrict__ in,
+                                                  const uint32_t *__restrict__ ksk,
+                                                  uint32_t *__restrict__ out, int tn, int B)
+{
+    constexpr int W = int(sizeof(TD) * 8);
+    constexpr int NDIG = DN * T;
+    static_assert(NDIG % 8 == 0, "digit positions are taken in pairs by four waves");
+    constexpr uint32_t mask = (1u << BB) - 1;
+    constexpr size_t nd = (size_t(1) << BB) - 1;
+    __shared__ uint16_t dig[NDIG];
+    __shared__ uint32_t part[4][NCH * 64];
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const TD *c = in + size_t(b) * (DN + 1);
+    const TD prec = TD(1) << (W - (1 + BB * T));
+    for (int e = threadIdx.x; e < NDIG; e += blockDim.x) {
+        const int i = e / T, j = e % T;
+        dig[e] = uint16_t(((c[i] + prec) >> (W - (j + 1) * BB)) & mask);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const size_t row = size_t(tn) + 1;
+    uint32_t acc[NCH];
+#pragma unroll
+    for (int j = 0; j < NCH; j++) acc[j] = 0;
+    for (int e = wave; e < NDIG; e += 8) {
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(uint32_t(dig[e]));
+        const uint32_t d1 = __builtin_amdgcn_readfirstlane(uint32_t(dig[e + 4]));
+        const uint32_t keep0 = d0 ? 0xffffffffu : 0u, keep1 = d1 ? 0xffffffffu : 0u;
+        const uint32_t *k0 = ksk + (size_t(e) * nd + (d0 ? d0 - 1 : 0)) * row + lane;
+        const uint32_t *k1 = ksk + (size_t(e + 4) * nd + (d1 ? d1 - 1 : 0)) * row + lane;
+        uint32_t v0[NCH], v1[NCH];
+#pragma unroll
+        for (int j = 0; j < NCH; j++) v0[j] = k0[64 * j];
+#pragma unroll
+        for (int j = 0; j < NCH; j++) v1[j] = k1[64 * j];
+#pragma unroll
+        for (int j = 0; j < NCH; j++) acc[j] -= (v0[j] & keep0) + (v1[j] & keep1);
+    }
+#pragma unroll
+    for (int j = 0; j < NCH; j++) part[wave][lane + 64 * j] = acc[j];
+    __syncthreads();
+    uint32_t body;
+    if constexpr (W == 32)
+        body = uint32_t(c[DN]);
+    else
+        body = uint32_t((c[DN] + (TD(1) << (W - 33))) >> (W - 32));
+    uint32_t *o = out + size_t(b) * row;
+    for (int k = threadIdx.x; k < int(row); k += blockDim.x)
+        o[k] = (k == tn ? body : 0u) + part[0][k] + part[1][k] + part[2][k] + part[3][k];
+}
+
+// launch k_iks_rows with NCH = ceil((tn + 1) / 64), tn <= kMaxN0
+template <class TD, int DN, int T, int BB, int NCH = 1>
+void launch_iks_rows(const TD *in, const uint32_t *ksk, uint32_t *out, int tn, int B, hipStream_t s)
+{
+    if constexpr (NCH < (kMaxN0 + 1 + 63) / 64) {
+        if ((tn + 1 + 63) / 64 > NCH) return launch_iks_rows<TD, DN, T, BB, NCH + 1>(in, ksk, out, tn, B, s);
+    }
+    hipLaunchKernelGGL((k_iks_rows<TD, DN, T, BB, NCH>), dim3(B), dim3(256), 0, s, in, ksk, out,
+                       tn, B);
+}
+
 // Tiled form for small bases (lvl21: basebit 3, 7 key rows per digit
 // position): a workgroup switches C ciphertexts on one 64-column slice of the
 // output, so the 7 candidate rows of a position are read by the C
@@ -281,12 +356,12 @@ hipError_t launch_iks(int kind, const void *in, const uint32_t *ksk, uint32_t *o
     if (B <= 0) return hipSuccess;
     switch (kind) {
     case 10:
-        hipLaunchKernelGGL((k_iks<uint32_t, Lvl1::N, KS10::t, KS10::basebit>), dim3(B), dim3(256),
-                           0, s, static_cast<const uint32_t *>(in), ksk, out, tn, B);
+        launch_iks_rows<uint32_t, Lvl1::N, KS10::t, KS10::basebit>(
+            static_cast<const uint32_t *>(in), ksk, out, tn, B, s);
         break;
     case 20:
-        hipLaunchKernelGGL((k_iks<uint64_t, Lvl2::N, KS20::t, KS20::basebit>), dim3(B), dim3(256),
-                           0, s, static_cast<const uint64_t *>(in), ksk, out, tn, B);
+        launch_iks_rows<uint64_t, Lvl2::N, KS20::t, KS20::basebit>(
+            static_cast<const uint64_t *>(in), ksk, out, tn, B, s);
         break;
     case 21: {
         constexpr int C = 32;

@@ -114,12 +114,10 @@ int fhe_keygen_ksk_dev(fhe_ctx *ctx, int kind, uint64_t seed, const void *s_dom,
                   kind);
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     const uint64_t records = uint64_t(g.dn) * g.t * ((uint64_t(1) << g.basebit) - 1);
-    const size_t key_bytes = fhe_ksk_words(&ctx->p, kind) * sizeof(uint32_t);
     uint32_t *dst = ksk_dev;
     if (!dst) {
-        const int slot = ksk_slot(kind);
-        if (!ctx->ksk[slot]) HIP_TRY(hipMalloc(&ctx->ksk[slot], key_bytes), "KSK allocation");
-        dst = ctx->ksk[slot];
+        if (int rc = alloc_ksk(ctx, kind)) return rc;
+        dst = ctx->ksk[ksk_slot(kind)];
     }
     // host side: the domain key as 32-bit words (a lvl2 key is 0/1 in 64
     // bits; the host generator truncates the same way) and the noise words

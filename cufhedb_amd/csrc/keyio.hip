@@ -321,7 +321,7 @@ int load_ksk_stream(fhe_ctx *ctx, File &f, int ks_kind, uint64_t bytes)
 {
     const int slot = ksk_slot(ks_kind);
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
-    if (!ctx->ksk[slot]) HIP_TRY(hipMalloc(&ctx->ksk[slot], bytes), "KSK allocation");
+    if (int rc = alloc_ksk(ctx, ks_kind)) return rc;
     return stream_to_device(ctx, f, ctx->ksk[slot], bytes, 4);
 }
 
