@@ -136,8 +136,10 @@ void launch_iks_rows(const TD *in, const uint32_t *ksk, uint32_t *out, int tn, i
 // position): a workgroup switches C ciphertexts on one 64-column slice of the
 // output, so the 7 candidate rows of a position are read by the C
 // ciphertexts from L1/L2 instead of once per ciphertext from HBM.  Each wave
-// takes C/4 ciphertexts (wave-uniform, so their a_i are scalar loads); each
-// lane owns one output column.
+// takes C/4 ciphertexts (wave-uniform, so their a_i and digits are scalar)
+// and each lane one column.  The body is branch-free so that the T x C/4
+// row reads of one a_i are all in flight together: a zero digit reads a
+// valid row and discards it, lanes past the last column read column tn.
 template <class TD, int DN, int T, int BB, int C>
 __global__ __launch_bounds__(256) void k_iks_tiled(const TD *__restrict__ in,
                                                    const uint32_t *__restrict__ ksk,
@@ -151,39 +153,49 @@ __global__ __launch_bounds__(256) void k_iks_tiled(const TD *__restrict__ in,
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int b0 = blockIdx.x * C + wv * PER;
     const size_t row = size_t(tn) + 1;
-    const bool colok = col <= tn;
     const TD prec = TD(1) << (W - (1 + BB * T));
     uint32_t acc[PER];
 #pragma unroll
-    for (int k = 0; k < PER; k++) {
-        const int b = b0 + k;
-        uint32_t body = 0;
-        if (b < B && col == tn) {
-            const TD v = in[size_t(b) * (DN + 1) + DN];
-            body = W == 32 ? uint32_t(v) : uint32_t((v + (TD(1) << (W - 33))) >> (W - 32));
-        }
-        acc[k] = body;
-    }
-    const uint32_t *kc = ksk + (colok ? col : 0);
+    for (int k = 0; k < PER; k++) acc[k] = 0;
+    const uint32_t *kc = ksk + min(col, tn);
     for (int i = 0; i < DN; i++) {
         TD abar[PER];
 #pragma unroll
-        for (int k = 0; k < PER; k++)
-            abar[k] = (b0 + k < B) ? TD(in[size_t(b0 + k) * (DN + 1) + i] + prec) : TD(0);
+        for (int k = 0; k < PER; k++) {
+            const TD a = (b0 + k < B) ? in[size_t(b0 + k) * (DN + 1) + i] : TD(0);
+            abar[k] = TD(a + prec);
+        }
+        // all T x PER row reads of this a_i first, then the subtractions
+        uint32_t v[T][PER];
 #pragma unroll
         for (int j = 0; j < T; j++) {
             const uint32_t *kr = kc + (size_t(i) * T + j) * nd * row;
 #pragma unroll
             for (int k = 0; k < PER; k++) {
                 const uint32_t d = uint32_t(abar[k] >> (W - (j + 1) * BB)) & mask;
-                if (d != 0 && colok) acc[k] -= kr[(d - 1) * row];
+                v[j][k] = kr[(d ? d - 1 : 0) * row];
             }
         }
-    }
-    if (colok) {
 #pragma unroll
-        for (int k = 0; k < PER; k++)
-            if (b0 + k < B) out[size_t(b0 + k) * row + col] = acc[k];
+        for (int j = 0; j < T; j++)
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                const uint32_t d = uint32_t(abar[k] >> (W - (j + 1) * BB)) & mask;
+                acc[k] -= d ? v[j][k] : 0u;
+            }
+    }
+    if (col <= tn) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int b = b0 + k;
+            if (b >= B) continue;
+            uint32_t body = 0;
+            if (col == tn) {
+                const TD v = in[size_t(b) * (DN + 1) + DN];
+                body = W == 32 ? uint32_t(v) : uint32_t((v + (TD(1) << (W - 33))) >> (W - 32));
+            }
+            out[size_t(b) * row + col] = body + acc[k];
+        }
     }
 }
 
