@@ -55,6 +55,26 @@ __device__ __forceinline__ double digit1p(uint32_t u, int d)
 }
 __device__ __forceinline__ double digit1(uint32_t x, int d) { return digit1p(prep1(x), d); }
 
+// Digit planes: the l digit polynomials of one accumulator column, kept in
+// LDS between the column's rows as signed 6-bit values in bytes.  Lane L
+// owns 16 bytes of each plane, byte 2j for coefficient L + 64 j and byte
+// 2j + 1 for coefficient L + 64 j + 512 (the order of the transform's input
+// registers), so each lane writes and reads only its own slots.
+// pack four prepared words' digit-d fields (raw 6-bit, sign-extended on read)
+__device__ __forceinline__ uint32_t pack_digits4(const uint32_t *u, int d)
+{
+    const int sh = 32 - Lvl1::Bgbit * (d + 1);
+    uint32_t w = __builtin_amdgcn_ubfe(u[0], sh, Lvl1::Bgbit);
+    w |= __builtin_amdgcn_ubfe(u[1], sh, Lvl1::Bgbit) << 8;
+    w |= __builtin_amdgcn_ubfe(u[2], sh, Lvl1::Bgbit) << 16;
+    w |= __builtin_amdgcn_ubfe(u[3], sh, Lvl1::Bgbit) << 24;
+    return w;
+}
+__device__ __forceinline__ double plane_digit(uint32_t w, int byte)
+{
+    return double(int32_t(__builtin_amdgcn_sbfe(int32_t(w), 8 * byte, Lvl1::Bgbit)));
+}
+
 // load one TRGSW row (both output columns) of the device key layout
 __device__ __forceinline__ void load_row(const cd *row, int lane, cd b0[8], cd b1[8])
 {
@@ -73,6 +93,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 {
     __shared__ uint32_t acc[2 * N1];
     __shared__ cd tile[kFftLdsElems];
+    __shared__ uint4 planes[Lvl1::l][64];
     const int lane = threadIdx.x;
     const int ct = blockIdx.x;
     if (ct >= B) return;
@@ -101,25 +122,47 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 #pragma unroll
         for (int q = 0; q < 8; q++) fd0[q] = fd1[q] = cd{0.0, 0.0};
 #pragma unroll 1
-        for (int r = 0; r < Lvl1::rows; r++) {
-            const int c = r / Lvl1::l, d = r - c * Lvl1::l;
-            const uint32_t *poly = acc + c * N1;
-            // issue this row's key loads before the FFT so their latency
-            // overlaps the transform
-            cd b0[8], b1[8];
-            load_row(bki + r * 2 * kPolyCd, lane, b0, b1);
-            // digits of (X^abar - 1) acc_c, recomputed per row from LDS
-            // (cheaper in registers than holding the rotated polynomial)
-            cd v[8];
+        for (int c = 0; c < 2; c++) {
+            // digits of (X^abar - 1) acc_c for the column's l rows: rotation
+            // and preparation once per column, then into the digit planes
+            {
+                const uint32_t *poly = acc + c * N1;
+                // the rotation's index arithmetic is redone per column (an
+                // opaque copy of abar keeps it from being hoisted into
+                // registers held across the rows)
+                uint32_t ac = abar;
+                asm volatile("" : "+v"(ac));
+                uint32_t u[16];
 #pragma unroll
-            for (int j = 0; j < 8; j++)
-                v[j] = cd{digit1(rot_minus_one(poly, abar, lane + 64 * j), d),
-                          digit1(rot_minus_one(poly, abar, lane + 64 * j + 512), d)};
-            fft512_fwd(v, tile, lane, tw);
+                for (int j = 0; j < 8; j++) {
+                    u[2 * j] = prep1(rot_minus_one(poly, ac, lane + 64 * j));
+                    u[2 * j + 1] = prep1(rot_minus_one(poly, ac, lane + 64 * j + 512));
+                }
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                cmac(fd0[q], v[q], b0[q]);
-                cmac(fd1[q], v[q], b1[q]);
+                for (int d = 0; d < Lvl1::l; d++)
+                    planes[d][lane] = uint4{pack_digits4(u, d), pack_digits4(u + 4, d),
+                                            pack_digits4(u + 8, d), pack_digits4(u + 12, d)};
+            }
+#pragma unroll 1
+            for (int d = 0; d < Lvl1::l; d++) {
+                const int r = c * Lvl1::l + d;
+                // issue this row's key loads before the FFT so their latency
+                // overlaps the transform
+                cd b0[8], b1[8];
+                load_row(bki + r * 2 * kPolyCd, lane, b0, b1);
+                const uint4 w = planes[d][lane];
+                const uint32_t wd[4] = {w.x, w.y, w.z, w.w};
+                cd v[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    v[j] = cd{plane_digit(wd[j >> 1], 2 * (j & 1)),
+                              plane_digit(wd[j >> 1], 2 * (j & 1) + 1)};
+                fft512_fwd(v, tile, lane, tw);
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    cmac(fd0[q], v[q], b0[q]);
+                    cmac(fd1[q], v[q], b1[q]);
+                }
             }
         }
         // every lane's rotation reads are complete (the FFTs synchronised)
