@@ -368,7 +368,10 @@ def main():
             res["queries"] = run_queries(F, octx, osk, op, args.query_rows, args.query_reps)
         if args.mixed:
             from cufhedb_amd import mixed
-            _, res["mixed_stream"] = mixed.run(octx, osk, op, mixed.make_plan(2026, args.mixed))
+            plan = mixed.make_plan(2026, args.mixed)
+            _, seq = mixed.run(octx, osk, op, plan)
+            _, coal = mixed.run_coalesced(octx, osk, op, plan)
+            res["mixed_stream"] = {"sequential": seq, "coalesced": coal}
             print(f"bench: mixed stream: {res['mixed_stream']}", file=sys.stderr, flush=True)
         octx.close()
     if not args.no_cpu_baseline:

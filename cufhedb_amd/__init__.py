@@ -132,6 +132,12 @@ _SIGS = {
     "fhe_keyfile_load": (_i, [_vp, ctypes.c_char_p, _i]),
     "fhe_fastr_write": (_i, [ctypes.c_char_p, _u32, _vp, _vp, _vp]),
     "fhe_fastr_load": (_i, [_vp, ctypes.c_char_p, _vp]),
+    "fhe_sched_create": (_i, [_vp, _i, _vp]),
+    "fhe_sched_destroy": (_i, [_vp]),
+    "fhe_sched_submit_pbs1": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "fhe_sched_submit_cmp2": (_i, [_vp, _i, _vp, _vp, _vp, _u32, _i, _i, _vp]),
+    "fhe_sched_flush": (_i, [_vp]),
+    "fhe_sched_wait": (_i, [_vp, _u64, _vp]),
     "fhe_pbs_lvl01": (_i, [_vp, _vp, _vp, _i, _vp, _i]),
     "fhe_iks": (_i, [_vp, _i, _vp, _vp, _i]),
     "fhe_malloc": (_i, [_vp, _sz]),
@@ -275,6 +281,51 @@ def torus32_signed(x):
 
 
 # ---------------------------------------------------------------- device side
+class Scheduler:
+    """Coalescing scheduler over a context (fhe_sched_*): requests are
+    queued by submit_*, served by flush() in merged launches of up to
+    max_rows rows, and waited for by ticket."""
+
+    PBS1, CMP2 = 0, 1
+
+    def __init__(self, ctx, max_rows=16384):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        check(lib().fhe_sched_create(ctx.h, int(max_rows), ctypes.byref(self.h)))
+
+    def submit_pbs1(self, in_buf, tv_buf, out_buf, B):
+        t = ctypes.c_uint64()
+        check(lib().fhe_sched_submit_pbs1(self.h, in_buf.ptr, tv_buf.ptr, out_buf.ptr, int(B),
+                                          ctypes.byref(t)))
+        return t.value
+
+    def submit_cmp2(self, op, a_buf, b_buf, out_buf, bits, result_type, B):
+        t = ctypes.c_uint64()
+        check(lib().fhe_sched_submit_cmp2(self.h, int(op), a_buf.ptr, b_buf.ptr, out_buf.ptr,
+                                          int(bits), int(result_type), int(B), ctypes.byref(t)))
+        return t.value
+
+    def flush(self):
+        check(lib().fhe_sched_flush(self.h))
+
+    def wait(self, ticket):
+        """Milliseconds from the start of the serving flush to completion."""
+        ms = ctypes.c_float()
+        check(lib().fhe_sched_wait(self.h, ctypes.c_uint64(ticket), ctypes.byref(ms)))
+        return float(ms.value)
+
+    def close(self):
+        if self.h:
+            lib().fhe_sched_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceBuffer:
     """Device allocation owned by Python (fhe_malloc / fhe_free)."""
 

@@ -3,23 +3,29 @@
 
 The reference serves such a stream with one host thread per CUDA stream and
 one ciphertext per call (src/cuHEDB/gatebootstrap_gpu.cuh:17-25,
-test/HE3_Query/TPCH_Q6.cu:179-193).  Here every batch is one batched call on
-the context stream (a lvl1 batch is one fhe_pbs_lvl01_dev launch, a lvl2
-batch one fhe_compare_dev chain), so the device sees few, large launches and
-no host round trip inside a batch.  Each batch is timed with HIP events on
-the stream; results are decrypted and checked after the timed call.
+test/HE3_Query/TPCH_Q6.cu:179-193).  Two ways are measured here:
+
+* ``run``: every batch is one batched call on the context stream (a lvl1
+  batch is one fhe_pbs_lvl01_dev launch, a lvl2 batch one fhe_compare_dev
+  chain), timed with HIP events on its own;
+* ``run_coalesced``: the whole stream is submitted to the coalescing
+  scheduler (fhe_sched_*), which merges consecutive batches of one kind into
+  launches of up to ``max_rows`` rows; a batch's latency is the time from
+  the start of the flush to its completion.
+
+Results are decrypted and checked after the timed region in both.
 
 Plan (seeded): batch sizes clip(round(lognormal(ln 2048, sigma)), 256, 65536),
 a fraction `frac_lvl1` of the batches are Z_8 LUT PBS (lvl0 -> lvl1), the rest
-are x <= t range compares of 20-bit lvl2 values against one shared
-encrypted threshold.
+are x <= t range compares of 20-bit lvl2 values against one encrypted
+threshold per batch.
 """
 
 import ctypes
 
 import numpy as np
 
-from . import (CMP_LE, LOGIC, LVL1_N, DeviceBuffer, check, lib, tlwe_encrypt_u32,
+from . import (CMP_LE, LOGIC, LVL1_N, DeviceBuffer, Scheduler, check, lib, tlwe_encrypt_u32,
                tlwe_encrypt_u64, tlwe_phase_u32)
 
 
@@ -38,60 +44,105 @@ def make_plan(seed, nbatches=200, sigma=1.0, frac_lvl1=0.7, max_size=65536):
     return [(str(k), int(s)) for k, s in zip(kinds, sizes)]
 
 
-def run(ctx, sk, params, plan, seed=5, bits=20):
-    """Runs the plan; returns per-batch records and a summary."""
-    L = lib()
-    rng = np.random.default_rng(seed)
-    records = []
-    for bi, (kind, B) in enumerate(plan):
+class _Batch:
+    """Device inputs / outputs and the plaintext expectation of one batch."""
+
+    def __init__(self, bi, kind, B, sk, params, rng, bits):
+        self.kind, self.B = kind, B
         if kind == "pbs1":
             m = rng.integers(0, 8, B)
             lut = rng.integers(0, 8, 8)
             c0 = tlwe_encrypt_u32(10_000 + bi, 0, sk.s0, (m << 28).astype(np.uint32),
                                   params.alpha0)
-            d_in = DeviceBuffer.from_array(c0)
-            d_tv = DeviceBuffer.from_array(z8_testvector(lut))
-            d_out = DeviceBuffer(B * (LVL1_N + 1) * 4)
-            call = (lambda: check(L.fhe_pbs_lvl01_dev(ctx.h, d_in.ptr, d_tv.ptr, 0, d_out.ptr,
-                                                       B, None)))
-            bufs = [d_in, d_tv, d_out]
+            self.inp = DeviceBuffer.from_array(c0)
+            self.aux = DeviceBuffer.from_array(z8_testvector(lut))
+            self.expect = lut[m]
         else:
             x = rng.integers(0, 1 << bits, B)
             t = int(rng.integers(0, 1 << bits))
             sc = np.uint64(64 - bits - 1)
-            d_in = DeviceBuffer.from_array(tlwe_encrypt_u64(20_000 + bi, 0, sk.s2,
-                                                            x.astype(np.uint64) << sc,
-                                                            params.alpha2))
-            d_t = DeviceBuffer.from_array(tlwe_encrypt_u64(30_000 + bi, 0, sk.s2,
-                                                           np.array([t], np.uint64) << sc,
-                                                           params.alpha2))
-            d_out = DeviceBuffer(B * (LVL1_N + 1) * 4)
-            call = (lambda: check(L.fhe_compare_dev(ctx.h, 2, CMP_LE, d_in.ptr, d_t.ptr, 1,
-                                                     d_out.ptr, bits, LOGIC, B, None)))
-            bufs = [d_in, d_t, d_out]
+            self.inp = DeviceBuffer.from_array(tlwe_encrypt_u64(20_000 + bi, 0, sk.s2,
+                                                                x.astype(np.uint64) << sc,
+                                                                params.alpha2))
+            self.aux = DeviceBuffer.from_array(tlwe_encrypt_u64(30_000 + bi, 0, sk.s2,
+                                                                np.array([t], np.uint64) << sc,
+                                                                params.alpha2))
+            self.expect = x <= t
+        self.out = DeviceBuffer(B * (LVL1_N + 1) * 4)
+
+    def result(self):
+        return self.out.to_array(np.uint32, (self.B, LVL1_N + 1))
+
+    def errors(self, sk):
+        ph = tlwe_phase_u32(self.result(), sk.s1).view(np.int32).astype(np.int64)
+        if self.kind == "pbs1":
+            return int(np.sum(np.round(ph / (1 << 28)).astype(np.int64) % 16 != self.expect))
+        return int(np.sum((ph > 0) != self.expect))
+
+    @property
+    def bootstraps(self):
+        # a 21-bit lvl2 HomMSB chain is 6 lvl2 + 3 lvl1 bootstraps per row
+        return self.B if self.kind == "pbs1" else 9 * self.B
+
+    def free(self):
+        for b in (self.inp, self.aux, self.out):
+            b.free()
+
+
+def run(ctx, sk, params, plan, seed=5, bits=20):
+    """Each batch one call, timed alone; returns per-batch records and a summary."""
+    L = lib()
+    rng = np.random.default_rng(seed)
+    records = []
+    for bi, (kind, B) in enumerate(plan):
+        b = _Batch(bi, kind, B, sk, params, rng, bits)
         ctx.sync()
         check(L.fhe_timer_start(ctx.h, None))
-        call()
+        if kind == "pbs1":
+            check(L.fhe_pbs_lvl01_dev(ctx.h, b.inp.ptr, b.aux.ptr, 0, b.out.ptr, B, None))
+        else:
+            check(L.fhe_compare_dev(ctx.h, 2, CMP_LE, b.inp.ptr, b.aux.ptr, 1, b.out.ptr, bits,
+                                    LOGIC, B, None))
         ms = ctypes.c_float()
         check(L.fhe_timer_stop(ctx.h, None, ctypes.byref(ms)))
-        out = d_out.to_array(np.uint32, (B, LVL1_N + 1))
-        ph = tlwe_phase_u32(out, sk.s1).view(np.int32).astype(np.int64)
-        if kind == "pbs1":
-            errors = int(np.sum(np.round(ph / (1 << 28)).astype(np.int64) % 16 != lut[m]))
-            pbs = B
-        else:
-            errors = int(np.sum((ph > 0) != (x <= t)))
-            pbs = 9 * B  # 6 lvl2 + 3 lvl1 bootstraps per row (21-bit HomMSB chain)
-        records.append({"kind": kind, "size": B, "ms": float(ms.value), "errors": errors,
-                        "bootstraps": pbs})
-        for b in bufs:
-            b.free()
+        records.append({"kind": kind, "size": B, "ms": float(ms.value), "errors": b.errors(sk),
+                        "bootstraps": b.bootstraps})
+        b.free()
     return records, summarize(records)
 
 
-def summarize(records):
+def run_coalesced(ctx, sk, params, plan, seed=5, bits=20, max_rows=16384):
+    """The whole plan through the coalescing scheduler in one flush (inputs
+    prepared on the device first); returns per-batch records and a summary
+    whose device_ms is the flush's span."""
+    rng = np.random.default_rng(seed)
+    batches = [_Batch(bi, kind, B, sk, params, rng, bits) for bi, (kind, B) in enumerate(plan)]
+    sched = Scheduler(ctx, max_rows)
+    ctx.sync()
+    tickets = []
+    for b in batches:
+        if b.kind == "pbs1":
+            tickets.append(sched.submit_pbs1(b.inp, b.aux, b.out, b.B))
+        else:
+            tickets.append(sched.submit_cmp2(CMP_LE, b.inp, b.aux, b.out, bits, LOGIC, b.B))
+    sched.flush()
+    lat = [sched.wait(t) for t in tickets]
+    sched.close()
+    records = [{"kind": b.kind, "size": b.B, "ms": ms, "errors": b.errors(sk),
+                "bootstraps": b.bootstraps} for b, ms in zip(batches, lat)]
+    for b in batches:
+        b.free()
+    out = summarize(records, span_ms=max(lat) if lat else 0.0)
+    out["max_rows"] = max_rows
+    return records, out
+
+
+def summarize(records, span_ms=None):
+    """Per-kind rows/s and batch latency percentiles.  span_ms: wall span of
+    a coalesced run (batches overlap in time); None = batches ran one after
+    another, so the span is the sum of their times."""
     lat = np.array([r["ms"] for r in records])
-    total_ms = float(lat.sum())
+    total_ms = float(lat.sum()) if span_ms is None else float(span_ms)
     out = {
         "batches": len(records),
         "device_ms": total_ms,
@@ -103,14 +154,18 @@ def summarize(records):
         rs = [r for r in records if r["kind"] == kind]
         if not rs:
             continue
-        ms = sum(r["ms"] for r in rs)
         rows = sum(r["size"] for r in rs)
-        out[kind] = {"batches": len(rs), "rows": rows, "device_ms": ms,
-                     "rows_per_s": rows / (ms / 1e3),
-                     "p50_batch_ms": float(np.percentile([r["ms"] for r in rs], 50)),
-                     "p99_batch_ms": float(np.percentile([r["ms"] for r in rs], 99))}
+        entry = {"batches": len(rs), "rows": rows,
+                 "p50_batch_ms": float(np.percentile([r["ms"] for r in rs], 50)),
+                 "p99_batch_ms": float(np.percentile([r["ms"] for r in rs], 99))}
+        if span_ms is None:
+            ms = sum(r["ms"] for r in rs)
+            entry["device_ms"] = ms
+            entry["rows_per_s"] = rows / (ms / 1e3)
+        out[kind] = entry
     out["bootstraps_per_s"] = sum(r["bootstraps"] for r in records) / (total_ms / 1e3)
+    out["rows_per_s"] = sum(r["size"] for r in records) / (total_ms / 1e3)
     return out
 
 
-__all__ = ["make_plan", "run", "summarize", "z8_testvector"]
+__all__ = ["make_plan", "run", "run_coalesced", "summarize", "z8_testvector"]

@@ -271,6 +271,35 @@ int fhe_compare_dev(fhe_ctx *ctx, int level, int op, const void *a_dev, const vo
 int fhe_hom_not_dev(fhe_ctx *ctx, int level, const void *in_dev, void *out_dev, int B,
                     void *stream);
 
+/* ---- coalescing scheduler (mixed lvl1 / lvl2 request stream) ------------ */
+/* Replaces the reference's serving pattern for a stream of independent
+ * requests -- one host thread per CUDA stream, one ciphertext per call
+ * (src/cuHEDB/gatebootstrap_gpu.cuh:17-25, test/HE3_Query/TPCH_Q6.cu:179-193)
+ * -- for the two request kinds of the mixed workload: lvl1 LUT bootstraps
+ * (fhe_pbs_lvl01_dev with one test vector per request) and lvl2 comparisons
+ * (fhe_compare_dev level 2 against one threshold ciphertext per request).
+ * fhe_sched_flush serves everything submitted so far, in order, merging
+ * consecutive requests of one kind (and one op / plain_bits / result_type)
+ * into batched launches of up to max_rows rows through device staging
+ * buffers; a request of more than max_rows rows runs alone.  Buffers must
+ * stay valid until fhe_sched_wait returns for the request.  One host thread
+ * per scheduler; the scheduler uses its context's stream. */
+typedef struct fhe_sched fhe_sched;
+enum fhe_sched_kind { FHE_SCHED_PBS1 = 0, FHE_SCHED_CMP2 = 1 };
+int fhe_sched_create(fhe_ctx *ctx, int max_rows, fhe_sched **out);
+int fhe_sched_destroy(fhe_sched *s);
+/* in_dev: B lvl0 TLWEs; tv_dev: one test vector (N words); out_dev: B lvl1 TLWEs */
+int fhe_sched_submit_pbs1(fhe_sched *s, const uint32_t *in_dev, const uint32_t *tv_dev,
+                          uint32_t *out_dev, int B, uint64_t *ticket);
+/* a_dev: B lvl2 TLWEs; b_dev: one lvl2 TLWE; out_dev: B lvl1 TLWEs (a op b) */
+int fhe_sched_submit_cmp2(fhe_sched *s, int op, const uint64_t *a_dev, const uint64_t *b_dev,
+                          uint32_t *out_dev, uint32_t plain_bits, int result_type, int B,
+                          uint64_t *ticket);
+int fhe_sched_flush(fhe_sched *s);
+/* Blocks until the request is done; ms (may be NULL) = device time from the
+ * start of the flush that served it to its completion. */
+int fhe_sched_wait(fhe_sched *s, uint64_t ticket, float *ms);
+
 /* ---- key material files ------------------------------------------------- */
 /* The reference's key files: TFHEpp cereal PortableBinary archives written by
  * HEDB::generate_*_key and read by HEDB::load_*_key

@@ -26,3 +26,58 @@ def test_mixed_plan_shape():
     assert sizes.min() >= 256 and sizes.max() <= 65536
     frac = np.mean([k == "pbs1" for k, _ in plan])
     assert 0.55 < frac < 0.85
+
+
+@pytest.mark.gpu
+def test_scheduler_matches_direct_calls(full672):
+    """Requests merged by the coalescing scheduler (fhe_sched_*) give the
+    same ciphertexts, bit for bit, as one direct call per request: each
+    ciphertext's bootstrap is independent of its batch.  Covers merging of
+    consecutive same-kind requests, a split at a change of comparison,
+    requests larger than max_rows (served alone) and single-row requests."""
+    k, ctx = full672
+    rng = np.random.default_rng(77)
+    spec = [("pbs1", 37, None), ("pbs1", 500, None), ("cmp2", 20, F.CMP_LE),
+            ("cmp2", 300, F.CMP_LE), ("cmp2", 40, F.CMP_GT), ("pbs1", 1, None),
+            ("pbs1", 700, None), ("pbs1", 1500, None), ("cmp2", 1100, F.CMP_LE),
+            ("pbs1", 64, None)]
+    batches = [mixed._Batch(i, kind, B, k.sk, k.params, rng, 20) for i, (kind, B, _) in
+               enumerate(spec)]
+    sched = F.Scheduler(ctx, max_rows=1024)
+    tickets = []
+    for b, (_, _, op) in zip(batches, spec):
+        if b.kind == "pbs1":
+            tickets.append(sched.submit_pbs1(b.inp, b.aux, b.out, b.B))
+        else:
+            tickets.append(sched.submit_cmp2(op, b.inp, b.aux, b.out, 20, F.LOGIC, b.B))
+    sched.flush()
+    lat = [sched.wait(t) for t in tickets]
+    assert all(ms > 0 for ms in lat)
+    with pytest.raises(F.FheError):
+        sched.wait(tickets[0])  # already waited for
+    sched.close()
+    L = F.lib()
+    for b, (_, _, op) in zip(batches, spec):
+        got = b.result()
+        direct = F.DeviceBuffer(b.B * (F.LVL1_N + 1) * 4)
+        if b.kind == "pbs1":
+            F.check(L.fhe_pbs_lvl01_dev(ctx.h, b.inp.ptr, b.aux.ptr, 0, direct.ptr, b.B, None))
+        else:
+            F.check(L.fhe_compare_dev(ctx.h, 2, op, b.inp.ptr, b.aux.ptr, 1, direct.ptr, 20,
+                                      F.LOGIC, b.B, None))
+        ctx.sync()
+        want = direct.to_array(np.uint32, (b.B, F.LVL1_N + 1))
+        direct.free()
+        assert np.array_equal(got, want), (b.kind, b.B, op)
+        if op != F.CMP_GT:
+            assert b.errors(k.sk) == 0
+        b.free()
+
+
+@pytest.mark.gpu
+def test_mixed_stream_coalesced(full672):
+    k, ctx = full672
+    plan = mixed.make_plan(seed=557, nbatches=12, sigma=0.8, frac_lvl1=0.6, max_size=2048)
+    records, summary = mixed.run_coalesced(ctx, k.sk, k.params, plan, seed=558, max_rows=4096)
+    print("coalesced mixed stream:", summary)
+    assert summary["errors"] == 0 and len(records) == len(plan)
