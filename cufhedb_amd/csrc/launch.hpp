@@ -38,24 +38,18 @@ hipError_t launch_blind_rotate1_mfma(const uint32_t *in, int n0, const float *bk
                                      const uint32_t *tv, int tv_stride, uint32_t *out,
                                      bool sample_extract, int B, hipStream_t s);
 
-// ---- lvl2 (N = 2048, 64-bit torus), pbs_lvl2.hip ------------------------
-size_t bk2_device_elems(int n0);
-hipError_t launch_bk2_from_int(const uint64_t *bk_int, cd *bk_dev, int n0, hipStream_t s);
-hipError_t launch_bk2_from_fft(const double *bk_fft, cd *bk_dev, int n0, hipStream_t s);
-hipError_t launch_blind_rotate2(const uint32_t *in, int n0, const cd *bk, const uint64_t *tv,
-                                int tv_stride, uint64_t *out, bool sample_extract, int B,
-                                hipStream_t s);
-// four-waves-per-ciphertext lvl2 blind rotation (pbs_lvl2q.hip) and its key layout
+// ---- lvl2 (N = 2048, 64-bit torus) ----------------------------------------
+// four-waves-per-ciphertext blind rotation (pbs_lvl2q.hip) and its key layout
 size_t bk2q_device_elems(int n0);
 hipError_t launch_bk2q_from_int(const uint64_t *bk_int, cd *bk_dev, int n0, hipStream_t s);
 hipError_t launch_blind_rotate2q(const uint32_t *in, int n0, const cd *bk, const uint64_t *tv,
                                  int tv_stride, uint64_t *out, bool sample_extract, int B,
                                  hipStream_t s);
 hipError_t launch_ifft2q(const uint64_t *poly, double *fd, int B, hipStream_t s);
+// external product, TwistFFT and sample extraction (pbs_lvl2.hip)
 hipError_t launch_trgsw2_from_fft(const double *trgswfft, cd *trgsw_dev, hipStream_t s);
 hipError_t launch_extprod2(const uint64_t *trlwe, const cd *trgsw_dev, uint64_t *out, int B,
                            hipStream_t s);
-hipError_t launch_ifft2(const uint64_t *poly, double *fd, int B, hipStream_t s);
 hipError_t launch_fft2(const double *fd, uint64_t *poly, int B, hipStream_t s);
 hipError_t launch_sample_extract2(const uint64_t *trlwe, uint64_t *tlwe, int index, int B,
                                   hipStream_t s);

@@ -1,24 +1,23 @@
-// pbs_lvl2.hip -- lvl2 (N = 2048, 64-bit torus, l = 4, Bgbit = 9) kernels for
-// gfx950, FP64 path.
+// pbs_lvl2.hip -- lvl2 (N = 2048, 64-bit torus, l = 4, Bgbit = 9) batched
+// external product, TwistFFT and sample extraction for gfx950, FP64 path.
+// (The lvl2 blind rotation is k_blind_rotate2q in pbs_lvl2q.hip.)
 //
 // The twisted transform has 1024 complex points.  A workgroup of two waves
-// owns one ciphertext: one radix-2 decimation-in-frequency step across the
-// two waves (through LDS) splits the 1024-point DFT into two plain 512-point
-// DFTs, and each wave runs the wave-level FFT of fft64.hpp on its half:
+// owns one polynomial pair: one radix-2 decimation-in-frequency step across
+// the two waves (through LDS) splits the 1024-point DFT into two plain
+// 512-point DFTs, and each wave runs the wave-level FFT of fft64.hpp on its
+// half:
 //   t_n  = z_n w2^n              (w2 = exp(i pi/2048), z_n = a_n + i a_{n+1024})
 //   u0_n = t_n + t_{n+512},  u1_n = (t_n - t_{n+512}) W1024^n,   n < 512
 //   Z[2m + h] = DFT512(u_h)[m]
 // Wave h, lane L, register j holds point n = L + 64 j + 512 h in time and
-// Z[2 fd_index1024(lane, q) + h] in frequency.  The accumulator (2 x 2048
-// 64-bit words, 32 KB) lives in LDS and both waves share it.
+// Z[2 fd_index1024(lane, q) + h] in frequency.
 //
-// Reference semantics: BlindRotate / CMUX / external product / decomposition
-// as in pbs_lvl1.hip, instantiated for lvl2param
-// (thirdparty/TFHEpp/include/params/128bit.hpp:35-50); torus64 conversions
-// follow execute_reverse_torus64 / execute_direct_torus64
+// Reference semantics: external product / decomposition as in pbs_lvl1.hip,
+// instantiated for lvl2param (thirdparty/TFHEpp/include/params/128bit.hpp:35-50);
+// torus64 conversions follow execute_reverse_torus64 / execute_direct_torus64
 // (thirdparty/TFHEpp/thirdparties/spqlios/fft_processor_spqlios.cpp:88-93,182-221);
-// GPU counterpart GateBootstrappingTLWE2TLWEFFT<Lvl2>
-// (src/cuHEDB/fft_gpu/BlindRotate_gpu.cuh:534-670).
+// GPU counterpart ExternalProduct<Lvl2> (src/cuHEDB/fft_gpu/ExternalProduct_gpu.cuh:371-473).
 #include "fft64.hpp"
 #include "launch.hpp"
 #include "params.hpp"
@@ -132,14 +131,6 @@ __device__ __forceinline__ uint64_t trunc_torus64(double x)
     return (uint64_t(uint32_t(Hm)) << 32) | uint64_t(uint32_t(L));
 }
 
-__device__ __forceinline__ uint64_t rot_minus_one2(const uint64_t *poly, uint32_t a, int n)
-{
-    uint32_t idx = (uint32_t(n) - a) & (2u * N2 - 1);
-    uint64_t v = poly[idx & (N2 - 1)];
-    if (idx & N2) v = 0ull - v;
-    return v - poly[n];
-}
-
 // Gadget digits of x (DecompositionPolynomial, trgsw.cpp:21-39, lvl2):
 //   digit_d = ((x + offset) >> (64 - 9(d+1)) & 511) - 256.
 // prep2 adds the offset and flips the top bit of every 9-bit field, so each
@@ -162,87 +153,6 @@ __device__ __forceinline__ void load_row2(const cd *row, int lane, int h, cd b0[
     for (int q = 0; q < 8; q++) {
         b0[q] = row[h * kHalfCd + q * 64 + lane];
         b1[q] = row[kPoly2Cd + h * kHalfCd + q * 64 + lane];
-    }
-}
-
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_blind_rotate2(const uint32_t *__restrict__ in, int n0,
-                                                       const cd *__restrict__ bk,
-                                                       const uint64_t *__restrict__ tv,
-                                                       int tv_stride, uint64_t *__restrict__ out,
-                                                       int sample_extract, int B)
-{
-    __shared__ uint64_t acc[2 * N2];
-    __shared__ cd tile[2][kFftLdsElems];
-    const int lane = threadIdx.x & 63, h = threadIdx.x >> 6, tid = threadIdx.x;
-    const int ct = blockIdx.x;
-    if (ct >= B) return;
-    const uint32_t *c0 = in + size_t(ct) * (n0 + 1);
-    const uint64_t *tvp = tv + size_t(ct) * tv_stride;
-    const Tw2 tw = make_tw2(lane, h);
-    cd *xch = tile[h];
-    const cd *pxch = tile[1 - h];
-
-    const uint32_t bbar = 2u * N2 - (c0[n0] >> (32 - 1 - Lvl2::nbit));
-    for (int n = tid; n < N2; n += 128) {
-        acc[n] = 0;
-        uint32_t idx = (uint32_t(n) - bbar) & (2u * N2 - 1);
-        uint64_t v = tvp[idx & (N2 - 1)];
-        acc[N2 + n] = (idx & N2) ? 0ull - v : v;
-    }
-    __syncthreads();
-
-    constexpr uint32_t kRound = 1u << (32 - 2 - Lvl2::nbit);
-    for (int i = 0; i < n0; i++) {
-        const uint32_t abar = (c0[i] + kRound) >> (32 - 1 - Lvl2::nbit);
-        if (abar == 0) continue;
-        const cd *bki = bk + size_t(i) * Lvl2::rows * kRow2Cd;
-        cd fd0[8], fd1[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) fd0[q] = fd1[q] = cd{0.0, 0.0};
-#pragma unroll 1
-        for (int r = 0; r < Lvl2::rows; r++) {
-            const int c = r / Lvl2::l, d = r - c * Lvl2::l;
-            const uint64_t *poly = acc + c * N2;
-            cd v[8];
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int n = lane + 64 * j + 512 * h;
-                v[j] = cd{digit2(rot_minus_one2(poly, abar, n), d),
-                          digit2(rot_minus_one2(poly, abar, n + 1024), d)};
-            }
-            fft1024_fwd(v, xch, pxch, lane, h, tw);
-            const cd *row = bki + r * kRow2Cd + h * kHalfCd + lane;
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                cmac(fd0[q], v[q], row[q * 64]);
-                cmac(fd1[q], v[q], row[kPoly2Cd + q * 64]);
-            }
-        }
-        // every rotation read of acc is complete (the transforms synchronised)
-        fft1024_inv(fd0, xch, pxch, lane, h, tw);
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int n = lane + 64 * j + 512 * h;
-            acc[n] += trunc_torus64(fd0[j].x * (1.0 / 1024));
-            acc[n + 1024] += trunc_torus64(fd0[j].y * (1.0 / 1024));
-        }
-        fft1024_inv(fd1, xch, pxch, lane, h, tw);
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int n = lane + 64 * j + 512 * h;
-            acc[N2 + n] += trunc_torus64(fd1[j].x * (1.0 / 1024));
-            acc[N2 + n + 1024] += trunc_torus64(fd1[j].y * (1.0 / 1024));
-        }
-        __syncthreads();
-    }
-
-    if (sample_extract) {
-        uint64_t *o = out + size_t(ct) * (N2 + 1);
-        for (int n = tid; n < N2; n += 128) o[n] = n == 0 ? acc[0] : 0ull - acc[N2 - n];
-        if (tid == 0) o[N2] = acc[N2];
-    } else {
-        uint64_t *o = out + size_t(ct) * 2 * N2;
-        for (int n = tid; n < 2 * N2; n += 128) o[n] = acc[n];
     }
 }
 
@@ -292,27 +202,7 @@ __global__ __launch_bounds__(128) void k_extprod2(const uint64_t *__restrict__ t
     }
 }
 
-// torus64 polynomials -> device FD layout [p][h][q][lane] (one workgroup each)
-__global__ __launch_bounds__(128) void k_poly_to_devfd2(const uint64_t *__restrict__ poly,
-                                                         cd *__restrict__ fd, int npoly)
-{
-    __shared__ cd tile[2][kFftLdsElems];
-    const int lane = threadIdx.x & 63, h = threadIdx.x >> 6;
-    const int p = blockIdx.x;
-    if (p >= npoly) return;
-    const uint64_t *a = poly + size_t(p) * N2;
-    const Tw2 tw = make_tw2(lane, h);
-    cd v[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const int n = lane + 64 * j + 512 * h;
-        v[j] = cd{double(int64_t(a[n])), double(int64_t(a[n + 1024]))};
-    }
-    fft1024_fwd(v, tile[h], tile[1 - h], lane, h, tw);
-#pragma unroll
-    for (int q = 0; q < 8; q++) fd[size_t(p) * kPoly2Cd + h * kHalfCd + q * 64 + lane] = v[q];
-}
-
+// bit reversal over 10 bits: spqlios order of the 1024 FD points
 __host__ __device__ constexpr int rev10(int x)
 {
     int r = 0;
@@ -335,31 +225,6 @@ __global__ void k_spq_to_devfd2(const double *__restrict__ spq, cd *__restrict__
     const int h = e / kHalfCd, q = (e % kHalfCd) >> 6, s = e & 63;
     const int idx = rev10(fd_index2048(h, s, q));
     fd[gid] = cd{spq[p * N2 + idx], spq[p * N2 + idx + 1024]};
-}
-
-__global__ __launch_bounds__(128) void k_ifft2(const uint64_t *__restrict__ poly,
-                                                double *__restrict__ fd, int B)
-{
-    __shared__ cd tile[2][kFftLdsElems];
-    const int lane = threadIdx.x & 63, h = threadIdx.x >> 6;
-    const int p = blockIdx.x;
-    if (p >= B) return;
-    const uint64_t *a = poly + size_t(p) * N2;
-    const Tw2 tw = make_tw2(lane, h);
-    cd v[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const int n = lane + 64 * j + 512 * h;
-        v[j] = cd{double(int64_t(a[n])), double(int64_t(a[n + 1024]))};
-    }
-    fft1024_fwd(v, tile[h], tile[1 - h], lane, h, tw);
-    double *o = fd + size_t(p) * N2;
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const int idx = rev10(fd_index2048(h, lane, q));
-        o[idx] = v[q].x;
-        o[idx + 1024] = v[q].y;
-    }
 }
 
 __global__ __launch_bounds__(128) void k_fft2(const double *__restrict__ fd,
@@ -409,33 +274,6 @@ inline unsigned grid1d(size_t n, unsigned block) { return unsigned((n + block - 
 
 }  // namespace
 
-size_t bk2_device_elems(int n0) { return size_t(n0) * Lvl2::rows * kRow2Cd; }
-
-hipError_t launch_bk2_from_int(const uint64_t *bk_int, cd *bk_dev, int n0, hipStream_t s)
-{
-    const int npoly = n0 * Lvl2::rows * 2;
-    hipLaunchKernelGGL(k_poly_to_devfd2, dim3(npoly), dim3(128), 0, s, bk_int, bk_dev, npoly);
-    return hipGetLastError();
-}
-
-hipError_t launch_bk2_from_fft(const double *bk_fft, cd *bk_dev, int n0, hipStream_t s)
-{
-    const int npoly = n0 * Lvl2::rows * 2;
-    hipLaunchKernelGGL(k_spq_to_devfd2, dim3(grid1d(size_t(npoly) * kPoly2Cd, 256)), dim3(256), 0,
-                       s, bk_fft, bk_dev, npoly);
-    return hipGetLastError();
-}
-
-hipError_t launch_blind_rotate2(const uint32_t *in, int n0, const cd *bk, const uint64_t *tv,
-                                int tv_stride, uint64_t *out, bool sample_extract, int B,
-                                hipStream_t s)
-{
-    if (B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_blind_rotate2, dim3(B), dim3(128), 0, s, in, n0, bk, tv, tv_stride, out,
-                       int(sample_extract), B);
-    return hipGetLastError();
-}
-
 hipError_t launch_trgsw2_from_fft(const double *trgswfft, cd *trgsw_dev, hipStream_t s)
 {
     const int npoly = Lvl2::rows * 2;
@@ -449,13 +287,6 @@ hipError_t launch_extprod2(const uint64_t *trlwe, const cd *trgsw_dev, uint64_t 
 {
     if (B <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_extprod2, dim3(B), dim3(128), 0, s, trlwe, trgsw_dev, out, B);
-    return hipGetLastError();
-}
-
-hipError_t launch_ifft2(const uint64_t *poly, double *fd, int B, hipStream_t s)
-{
-    if (B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ifft2, dim3(B), dim3(128), 0, s, poly, fd, B);
     return hipGetLastError();
 }
 

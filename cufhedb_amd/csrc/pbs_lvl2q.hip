@@ -1,12 +1,12 @@
 // pbs_lvl2q.hip -- lvl2 (N = 2048, 64-bit torus, l = 4, Bgbit = 9) blind
 // rotation with four waves per ciphertext.
 //
-// Same algorithm as k_blind_rotate2 in pbs_lvl2.hip (reference:
-// thirdparty/TFHEpp/include/gatebootstrapping.hpp:15-43 with lvl02param,
-// src/detwfa.cpp:21-30, src/trgsw.cpp:9-103; GPU counterpart
-// src/cuHEDB/fft_gpu/BlindRotate_gpu.cuh:534-670), laid out for occupancy:
-// each wave holds 4 of the 1024 complex points per lane instead of 8, so a
-// wave needs about half the registers and three waves fit per SIMD (the
+// Reference: thirdparty/TFHEpp/include/gatebootstrapping.hpp:15-43 with
+// lvl02param, src/detwfa.cpp:21-30, src/trgsw.cpp:9-103; GPU counterpart
+// src/cuHEDB/fft_gpu/BlindRotate_gpu.cuh:534-670.  Laid out for occupancy:
+// each wave holds 4 of the 1024 complex points per lane (a two-wave layout
+// with 8 per lane needs ~320 registers, one wave per SIMD), so a wave needs
+// about half the registers and three waves fit per SIMD (the
 // accumulator, 2 x 2048 x 64-bit = 32 KB, stays one copy per ciphertext in
 // LDS).  The twisted transform Z[m] = sum_n z_n w2^{n(4m+1)},
 // w2 = exp(i pi/2048), factors as
