@@ -51,6 +51,7 @@ struct Req {
     uint64_t ticket;
     hipEvent_t start = nullptr, done = nullptr;  // start: the flush that serves it began
     bool flushed = false;
+    bool failed = false;  // its flush stopped on an error before its result was complete
 };
 
 #define SCHED_TRY(expr)                    \
@@ -163,6 +164,37 @@ int run_alone(fhe_sched *s, Req &r, hipStream_t st)
     return FHE_OK;
 }
 
+// serves every request not yet flushed, in order
+int flush_pending(fhe_sched *s)
+{
+    const hipStream_t st = s->ctx->stream;
+    for (Req &r : s->reqs)
+        if (!r.flushed) HIP_TRY(hipEventRecord(r.start, st), "hipEventRecord");
+    std::vector<Req *> group;
+    int rows = 0;
+    auto close_group = [&]() -> int {
+        if (group.empty()) return FHE_OK;
+        const int rc = run_group(s, group, rows, st);
+        group.clear();
+        rows = 0;
+        return rc;
+    };
+    for (Req &r : s->reqs) {
+        if (r.flushed) continue;
+        r.flushed = true;
+        if (r.B > s->max_rows) {
+            SCHED_TRY(close_group());
+            SCHED_TRY(run_alone(s, r, st));
+            continue;
+        }
+        if (!group.empty() && (!same_group(*group[0], r) || rows + r.B > s->max_rows))
+            SCHED_TRY(close_group());
+        group.push_back(&r);
+        rows += r.B;
+    }
+    return close_group();
+}
+
 int submit(fhe_sched *s, Req r, uint64_t *ticket)
 {
     FHE_CHECK_ARG(s && ticket && r.in && r.aux && r.out && r.B >= 0, "fhe_sched_submit: bad argument");
@@ -246,32 +278,20 @@ int fhe_sched_flush(fhe_sched *s)
 {
     FHE_CHECK_ARG(s, "fhe_sched_flush: null scheduler");
     HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
-    const hipStream_t st = s->ctx->stream;
-    for (Req &r : s->reqs)
-        if (!r.flushed) HIP_TRY(hipEventRecord(r.start, st), "hipEventRecord");
-    std::vector<Req *> group;
-    int rows = 0;
-    auto close_group = [&]() -> int {
-        if (group.empty()) return FHE_OK;
-        const int rc = run_group(s, group, rows, st);
-        group.clear();
-        rows = 0;
-        return rc;
-    };
-    for (Req &r : s->reqs) {
-        if (r.flushed) continue;
-        r.flushed = true;
-        if (r.B > s->max_rows) {
-            SCHED_TRY(close_group());
-            SCHED_TRY(run_alone(s, r, st));
-            continue;
-        }
-        if (!group.empty() && (!same_group(*group[0], r) || rows + r.B > s->max_rows))
-            SCHED_TRY(close_group());
-        group.push_back(&r);
-        rows += r.B;
+    std::vector<uint64_t> mine;
+    for (const Req &r : s->reqs)
+        if (!r.flushed) mine.push_back(r.ticket);
+    const int rc = flush_pending(s);
+    if (rc != FHE_OK) {
+        // results of this flush are incomplete: its requests fail on wait
+        for (Req &r : s->reqs)
+            for (uint64_t t : mine)
+                if (r.ticket == t) {
+                    r.flushed = true;
+                    r.failed = true;
+                }
     }
-    return close_group();
+    return rc;
 }
 
 int fhe_sched_wait(fhe_sched *s, uint64_t ticket, float *ms)
@@ -283,6 +303,14 @@ int fhe_sched_wait(fhe_sched *s, uint64_t ticket, float *ms)
         if (!r.flushed) {
             set_error("fhe_sched_wait: request %llu was not flushed", (unsigned long long)ticket);
             return FHE_ERR_ARG;
+        }
+        if (r.failed) {
+            set_error("fhe_sched_wait: the flush serving request %llu failed",
+                      (unsigned long long)ticket);
+            (void)hipEventDestroy(r.start);
+            (void)hipEventDestroy(r.done);
+            s->reqs.erase(s->reqs.begin() + long(i));
+            return FHE_ERR_DEVICE;
         }
         HIP_TRY(hipEventSynchronize(r.done), "hipEventSynchronize");
         if (ms) HIP_TRY(hipEventElapsedTime(ms, r.start, r.done), "hipEventElapsedTime");

@@ -111,3 +111,19 @@ def test_tlwe_encryption_decrypts(keys636, rng):
     e = O.tdiff32(F.tlwe_phase_u32(c, keys636.sk.s0), m)
     assert np.max(np.abs(e)) < 6 * 2.0 ** -15 * 2 ** 32
     assert np.array_equal(keys636.enc_lvl0(m), c)  # deterministic per (seed, index)
+
+
+def test_scheduler_and_keyfile_argument_checks():
+    """Argument errors of the scheduler and key-file entry points are
+    reported before any device work (no GPU needed)."""
+    L = F.lib()
+    out = ctypes.c_void_p()
+    assert L.fhe_sched_create(None, 16, ctypes.byref(out)) == F.FHE_ERR_ARG
+    assert L.fhe_sched_flush(None) == F.FHE_ERR_ARG
+    assert L.fhe_sched_wait(None, 1, None) == F.FHE_ERR_ARG
+    t = ctypes.c_uint64()
+    assert L.fhe_sched_submit_cmp2(None, 9, None, None, None, 20, 1, 4, ctypes.byref(t)) == \
+        F.FHE_ERR_ARG
+    assert L.fhe_keyfile_load(None, b"x", 1) == F.FHE_ERR_ARG
+    assert L.fhe_fastr_load(None, b"x", None) == F.FHE_ERR_ARG
+    assert L.fhe_keygen_ksk_dev(None, 10, 1, None, None, None) == F.FHE_ERR_ARG

@@ -81,3 +81,27 @@ def test_mixed_stream_coalesced(full672):
     records, summary = mixed.run_coalesced(ctx, k.sk, k.params, plan, seed=558, max_rows=4096)
     print("coalesced mixed stream:", summary)
     assert summary["errors"] == 0 and len(records) == len(plan)
+
+
+@pytest.mark.gpu
+def test_scheduler_failed_flush_fails_its_requests(keys636):
+    """A flush that stops on an error (here: no bootstrapping key loaded)
+    returns it, and waiting on its requests reports the failure instead of
+    returning incomplete results."""
+    ctx = F.Context(keys636.params)  # no keys
+    sched = F.Scheduler(ctx, max_rows=64)
+    n = 8
+    inp = F.DeviceBuffer(n * (keys636.params.n0 + 1) * 4)
+    tv = F.DeviceBuffer(F.LVL1_N * 4)
+    out = F.DeviceBuffer(n * (F.LVL1_N + 1) * 4)
+    F.check(F.lib().fhe_memset_dev(inp.ptr, 0, inp.nbytes))
+    F.check(F.lib().fhe_memset_dev(tv.ptr, 0, tv.nbytes))
+    t = sched.submit_pbs1(inp, tv, out, n)
+    with pytest.raises(F.FheError):
+        sched.flush()
+    with pytest.raises(F.FheError):
+        sched.wait(t)
+    sched.close()
+    for b in (inp, tv, out):
+        b.free()
+    ctx.close()
