@@ -231,3 +231,37 @@ def test_gpu_gates_match_reference_golden(rng):
     ph = F.tlwe_phase_u32(ctx.ide_gate(g["cv"], 27), k.sk.s1)
     assert np.max(np.abs(O.tdiff32(ph, g["ide"]))) <= tol
     ctx.close()
+
+
+@pytest.mark.parametrize("n0", [1, 63, 64, 65, 1024])
+def test_gpu_iks_to_lvl0_output_widths(n0):
+    """The whole-row key switch reads each key row in ceil((n0 + 1) / 64)
+    chunks of 64 columns: widths that fill the last chunk exactly (n0 = 63),
+    leave one column in it (64, 1024: 17 chunks, the largest lvl0
+    dimension), or are tiny (1) are bit-exact against the oracle for both
+    lvl1 -> lvl0 and (up to n0 = 65) lvl2 -> lvl0; every output row is
+    checked, including the last row of the batch (the padded key tail)."""
+    p = F.preset(F.PRESET_CFG636)
+    p.n0 = n0
+    sk = F.SecretKeys(p, 9000 + n0)
+    ctx = F.Context(p)
+    rng = np.random.default_rng(n0)
+    kinds = (10, 20) if n0 <= 65 else (10,)
+    for kind in kinds:
+        ksk = F.keygen_ksk(sk, kind, 9100 + n0 + kind)
+        ctx.load_ksk(kind, ksk)
+        B = 7
+        if kind == 10:
+            x = rng.integers(0, 2 ** 32, (B, F.LVL1_N + 1), dtype=np.uint64).astype(np.uint32)
+            x[-1, :] = 0xFFFFFFFF  # all-ones edge input
+            got = ctx.iks(10, x)
+            for b in range(B):
+                assert np.array_equal(got[b], O.iks32(x[b], ksk, n0)), (kind, b)
+        else:
+            x = rng.integers(0, 2 ** 63, (B, F.LVL2_N + 1), dtype=np.uint64) * np.uint64(2)
+            x[-1, :] = np.uint64(2 ** 64 - 1)
+            got = ctx.iks(20, x)
+            for b in range(B):
+                assert np.array_equal(got[b], O.iks64(x[b], ksk, n0, 2, 10)), (kind, b)
+        del ksk
+    ctx.close()
