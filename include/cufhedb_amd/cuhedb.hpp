@@ -26,6 +26,9 @@
 //   * Batched forms (the *_batch members) take B ciphertexts per call; that
 //     is the fast path.
 //   * Errors throw cufhedb_amd::Error (the reference ignores CUDA errors).
+//   cufhedb_amd::Scheduler            the per-stream_id serving loop of the
+//                                     query programs (TPCH_Q6.cu:179-193),
+//                                     as merged batched launches
 //   HEDB:: key files                  src/HEDB/utils/serialize.h:6-61
 //     load_secret_key, load_bk*/bkfft*/iksk*_key (from ./../evalkey/ by
 //     default, as there) -- the evaluation keys go straight into a Runtime
@@ -117,6 +120,45 @@ class Runtime {
   private:
     fhe_params params_{};
     fhe_ctx *ctx_ = nullptr;
+};
+
+// Coalescing scheduler over a Runtime (fhe_sched_*): queue device-buffer
+// requests, flush() serves them in merged full-GPU launches, wait() blocks
+// on one and returns its latency from the start of its flush (ms).
+class Scheduler {
+  public:
+    explicit Scheduler(Runtime &rt, int max_rows = 16384)
+    {
+        check(fhe_sched_create(rt.ctx(), max_rows, &s_), "fhe_sched_create");
+    }
+    ~Scheduler() { fhe_sched_destroy(s_); }
+    Scheduler(const Scheduler &) = delete;
+    Scheduler &operator=(const Scheduler &) = delete;
+
+    uint64_t submit_pbs1(const uint32_t *in_dev, const uint32_t *tv_dev, uint32_t *out_dev, int B)
+    {
+        uint64_t t = 0;
+        check(fhe_sched_submit_pbs1(s_, in_dev, tv_dev, out_dev, B, &t), "fhe_sched_submit_pbs1");
+        return t;
+    }
+    uint64_t submit_cmp2(int op, const uint64_t *a_dev, const uint64_t *b_dev, uint32_t *out_dev,
+                         uint32_t plain_bits, int result_type, int B)
+    {
+        uint64_t t = 0;
+        check(fhe_sched_submit_cmp2(s_, op, a_dev, b_dev, out_dev, plain_bits, result_type, B, &t),
+              "fhe_sched_submit_cmp2");
+        return t;
+    }
+    void flush() { check(fhe_sched_flush(s_), "fhe_sched_flush"); }
+    float wait(uint64_t ticket)
+    {
+        float ms = 0.f;
+        check(fhe_sched_wait(s_, ticket, &ms), "fhe_sched_wait");
+        return ms;
+    }
+
+  private:
+    fhe_sched *s_ = nullptr;
 };
 
 }  // namespace cufhedb_amd

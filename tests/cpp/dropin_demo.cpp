@@ -5,6 +5,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <random>
 #include <string>
@@ -67,6 +68,44 @@ int main()
         cufftlvl1.GateBootstrappingTLWE2TLWEFFT_st(out, c0, lvl1param::mu, 0);
         const bool pos = int32_t(phase(out.data(), s1.data(), 1024)) > 0;
         bad += pos != bool(bit);
+    }
+    // three requests of different sizes and test vectors through the
+    // coalescing scheduler (one merged launch); outputs checked per request
+    {
+        cufhedb_amd::Scheduler sched(rt, 64);
+        const int sizes[3] = {5, 17, 2};
+        const uint32_t us[3] = {lvl1param::mu, 1u << 28, 3u << 27};
+        cufhedb_amd::DeviceBuffer din[3], dtv[3], dout[3];
+        std::vector<std::vector<int>> bits(3);
+        uint64_t tk[3];
+        for (int q = 0; q < 3; q++) {
+            const int B = sizes[q];
+            std::vector<uint32_t> c(size_t(B) * (n0 + 1)), m(B), tvv(1024, us[q]);
+            for (int i = 0; i < B; i++) {
+                bits[q].push_back(int(gen() & 1));
+                m[i] = bits[q][i] ? (1u << 30) : (3u << 30);
+            }
+            cufhedb_amd::check(fhe_tlwe_encrypt_u32(40 + q, 0, s0.data(), n0, m.data(), p.alpha0,
+                                                    c.data(), B), "enc");
+            din[q].reset(c.size() * 4);
+            din[q].upload(c.data(), c.size() * 4);
+            dtv[q].reset(4096);
+            dtv[q].upload(tvv.data(), 4096);
+            dout[q].reset(size_t(B) * 1025 * 4);
+            tk[q] = sched.submit_pbs1(din[q].as<uint32_t>(), dtv[q].as<uint32_t>(),
+                                      dout[q].as<uint32_t>(), B);
+        }
+        sched.flush();
+        for (int q = 0; q < 3; q++) {
+            sched.wait(tk[q]);
+            std::vector<uint32_t> o(size_t(sizes[q]) * 1025);
+            dout[q].download(o.data(), o.size() * 4);
+            for (int i = 0; i < sizes[q]; i++) {
+                const int32_t ph = int32_t(phase(o.data() + size_t(i) * 1025, s1.data(), 1024));
+                const int32_t want = bits[q][i] ? int32_t(us[q]) : -int32_t(us[q]);
+                bad += std::abs(int64_t(ph) - want) > (1 << 25);
+            }
+        }
     }
     // ifft_st / fft_st round trip within one unit (TFHEpp/test/polymul.cpp:19-28)
     std::array<uint32_t, 1024> a, back;
