@@ -141,3 +141,50 @@ def test_gpu_blind_rotate_phase(keys672, ctx672, rng):
         assert np.max(np.abs(dphi)) <= PHASE_TOL
         # sample extract of the GPU accumulator equals the GPU PBS output
         assert np.array_equal(O.sample_extract32(acc[b]), ctx672.pbs_lvl01(c0[b:b + 1], tv)[0])
+
+
+def test_gpu_pbs_trivial_and_boundary_masks(keys672, ctx672, rng):
+    """Edge inputs of the blind rotation.
+    * Trivial ciphertexts (mask 0): every a-bar is 0, every CMUX step is the
+      reference's skip, so the output is exactly SampleExtract(0, X^-bbar tv):
+      a zero mask and body tv rotated -- bit-exact, independent of the FFT.
+    * Mask words on both sides of the a-bar rounding boundary (2^20 - 1 -> 0,
+      2^20 -> 1, and the top of the torus): phase against the oracle.
+    * An empty batch is a no-op."""
+    N = F.LVL1_N
+    n0 = keys672.n0
+    tv = z8_testvector(rng.integers(0, 8, 8))
+    bodies = np.array([0, 1, 1 << 21, (1 << 21) - 1, 0x80000000, 0xFFFFFFFF, 1234 << 21],
+                      np.uint32)
+    c0 = np.zeros((len(bodies), n0 + 1), np.uint32)
+    c0[:, n0] = bodies
+    out = ctx672.pbs_lvl01(c0, tv)
+    for b, body in enumerate(bodies):
+        bbar = (2 * N - (int(body) >> 21)) % (2 * N)
+        # coefficient 0 of X^bbar tv (negacyclic): tv[-bbar mod 2N], negated
+        # when the index wraps past N
+        idx = (0 - bbar) % (2 * N)
+        want = tv[idx] if idx < N else (0 - int(tv[idx - N])) % (1 << 32)
+        assert np.all(out[b, :N] == 0), b
+        assert int(out[b, N]) == int(want), b
+    # masks at the a-bar rounding boundary
+    m = rng.integers(0, 8, 4)
+    c = keys672.enc_lvl0((m * DELTA8).astype(np.uint32), index0=9100)
+    c[0, :n0] = (1 << 20) - 1
+    c[1, :n0] = 1 << 20
+    c[2, : n0 // 2] = 0xFFFFFFFF
+    c[3, ::3] = 0
+    got = ctx672.pbs_lvl01(c, tv)
+    for b in range(4):
+        ref = O.pbs32(c[b], keys672.bkfft, tv)
+        dphi = O.tdiff32(F.tlwe_phase_u32(got[b:b + 1], keys672.sk.s1)[0],
+                         F.tlwe_phase_u32(ref, keys672.sk.s1)[0])
+        assert abs(int(dphi)) <= PHASE_TOL, (b, int(dphi))
+    # empty batch
+    d_in = F.DeviceBuffer(4 * (n0 + 1))
+    d_tv = F.DeviceBuffer.from_array(tv)
+    d_out = F.DeviceBuffer(4 * (N + 1))
+    F.check(F.lib().fhe_pbs_lvl01_dev(ctx672.h, d_in.ptr, d_tv.ptr, 0, d_out.ptr, 0, None))
+    ctx672.sync()
+    for buf in (d_in, d_tv, d_out):
+        buf.free()

@@ -265,3 +265,24 @@ def test_gpu_iks_to_lvl0_output_widths(n0):
                 assert np.array_equal(got[b], O.iks64(x[b], ksk, n0, 2, 10)), (kind, b)
         del ksk
     ctx.close()
+
+
+def test_gpu_pbs_lvl02_trivial_ciphertexts(full672):
+    """Trivial lvl0 ciphertexts (mask 0) skip every CMUX step of the lvl2
+    blind rotation: the output is exactly SampleExtract(0, X^bbar tv) with
+    bbar = 2N - (b >> 20), a zero mask and body tv rotated (bit-exact)."""
+    k, ctx = full672
+    N = F.LVL2_N
+    rng = np.random.default_rng(2)
+    tv = rng.integers(0, 2 ** 63, N, dtype=np.uint64) * np.uint64(2)
+    bodies = np.array([0, 1, 1 << 20, (1 << 20) - 1, 0x80000000, 0xFFFFFFFF, 777 << 20],
+                      np.uint32)
+    c0 = np.zeros((len(bodies), k.n0 + 1), np.uint32)
+    c0[:, k.n0] = bodies
+    out = ctx.pbs_lvl02(c0, tv)
+    for b, body in enumerate(bodies):
+        bbar = (2 * N - (int(body) >> 20)) % (2 * N)
+        idx = (0 - bbar) % (2 * N)
+        want = int(tv[idx]) if idx < N else (0 - int(tv[idx - N])) % (1 << 64)
+        assert np.all(out[b, :N] == 0), b
+        assert int(out[b, N]) == want, b
