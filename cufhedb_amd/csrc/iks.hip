@@ -140,7 +140,7 @@ void launch_iks_rows(const TD *in, const uint32_t *ksk, uint32_t *out, int tn, i
 // and each lane one column.  The body is branch-free so that the T x C/4
 // row reads of one a_i are all in flight together: a zero digit reads a
 // valid row and discards it, lanes past the last column read column tn.
-template <class TD, int DN, int T, int BB, int C>
+template <class TD, int DN, int T, int BB, int C, int TJ>
 __global__ __launch_bounds__(256) void k_iks_tiled(const TD *__restrict__ in,
                                                    const uint32_t *__restrict__ ksk,
                                                    uint32_t *__restrict__ out, int tn, int B)
@@ -165,24 +165,33 @@ __global__ __launch_bounds__(256) void k_iks_tiled(const TD *__restrict__ in,
             const TD a = (b0 + k < B) ? in[size_t(b0 + k) * (DN + 1) + i] : TD(0);
             abar[k] = TD(a + prec);
         }
-        // all T x PER row reads of this a_i first, then the subtractions
-        uint32_t v[T][PER];
+        // TJ x PER row reads of this a_i issued together, then the
+        // subtractions
 #pragma unroll
-        for (int j = 0; j < T; j++) {
-            const uint32_t *kr = kc + (size_t(i) * T + j) * nd * row;
+        for (int j0 = 0; j0 < T; j0 += TJ) {
+            uint32_t v[TJ][PER];
 #pragma unroll
-            for (int k = 0; k < PER; k++) {
-                const uint32_t d = uint32_t(abar[k] >> (W - (j + 1) * BB)) & mask;
-                v[j][k] = kr[(d ? d - 1 : 0) * row];
+            for (int jj = 0; jj < TJ; jj++) {
+                const int j = j0 + jj;
+                if (j >= T) break;
+                const uint32_t *kr = kc + (size_t(i) * T + j) * nd * row;
+#pragma unroll
+                for (int k = 0; k < PER; k++) {
+                    const uint32_t d = uint32_t(abar[k] >> (W - (j + 1) * BB)) & mask;
+                    v[jj][k] = kr[(d ? d - 1 : 0) * row];
+                }
+            }
+#pragma unroll
+            for (int jj = 0; jj < TJ; jj++) {
+                const int j = j0 + jj;
+                if (j >= T) break;
+#pragma unroll
+                for (int k = 0; k < PER; k++) {
+                    const uint32_t d = uint32_t(abar[k] >> (W - (j + 1) * BB)) & mask;
+                    acc[k] -= d ? v[jj][k] : 0u;
+                }
             }
         }
-#pragma unroll
-        for (int j = 0; j < T; j++)
-#pragma unroll
-            for (int k = 0; k < PER; k++) {
-                const uint32_t d = uint32_t(abar[k] >> (W - (j + 1) * BB)) & mask;
-                acc[k] -= d ? v[j][k] : 0u;
-            }
     }
     if (col <= tn) {
 #pragma unroll
@@ -376,9 +385,12 @@ hipError_t launch_iks(int kind, const void *in, const uint32_t *ksk, uint32_t *o
             static_cast<const uint64_t *>(in), ksk, out, tn, B, s);
         break;
     case 21: {
-        constexpr int C = 32;
+        // 32 ciphertexts per tile, all 10 digit rows of a mask word in flight
+        // (64-ciphertext tiles measured 20-120 % slower: register and SGPR
+        // pressure lower the occupancy more than the halved key traffic gains)
+        constexpr int C = 32, TJ = KS21::t;
         const dim3 grid(unsigned((B + C - 1) / C), unsigned((tn + 1 + 63) / 64));
-        hipLaunchKernelGGL((k_iks_tiled<uint64_t, Lvl2::N, KS21::t, KS21::basebit, C>), grid,
+        hipLaunchKernelGGL((k_iks_tiled<uint64_t, Lvl2::N, KS21::t, KS21::basebit, C, TJ>), grid,
                            dim3(256), 0, s, static_cast<const uint64_t *>(in), ksk, out, tn, B);
         break;
     }
