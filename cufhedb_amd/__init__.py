@@ -122,6 +122,8 @@ _SIGS = {
     "fhe_fwd_digits_lvl1_mfma_dev": (_i, [_vp, _vp, _vp, _i, _vp]),
     "fhe_compare_dev": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _u32, _i, _i, _vp]),
     "fhe_hom_not_dev": (_i, [_vp, _i, _vp, _vp, _i, _vp]),
+    "fhe_compare_digits_dev": (_i, [_vp, _i, _vp, _i, _i, _u64, _i, _vp, _i, _i, _vp]),
+    "fhe_digit_testvectors": (_i, [_i, _i, _i, _u64, _i, _vp, _vp, _vp]),
     "fhe_keyfile_params_of": (_i, [_vp, _vp]),
     "fhe_keyfile_payload_bytes": (_u64, [_i, _u32]),
     "fhe_keyfile_info": (_i, [ctypes.c_char_p, _i, _vp, _vp]),
@@ -606,6 +608,46 @@ def compare(ctx, level, op, a, b, plain_bits, result_type=LOGIC):
                                 result_type, B, None))
     ctx.sync()
     return dout.to_array(np.uint32, (B, LVL1_N + 1))
+
+
+def encrypt_digits(seed, index0, s2, values, digits, digit_bits, alpha):
+    """B integers as B x digits lvl2 TLWEs under the lvl2 key s2, least
+    significant digit first, digit v at v * 2^(63 - digit_bits) (the input
+    layout of compare_digits).  Noise words come from (seed, index0 + row)."""
+    v = np.ascontiguousarray(values, np.uint64).reshape(-1)
+    shifts = np.arange(digits, dtype=np.uint64) * np.uint64(digit_bits)
+    d = (v[:, None] >> shifts[None, :]) & np.uint64((1 << digit_bits) - 1)
+    m = d << np.uint64(63 - digit_bits)
+    ct = tlwe_encrypt_u64(seed, index0, s2, m.reshape(-1), alpha)
+    return ct.reshape(v.shape[0], digits, LVL2_N + 1)
+
+
+def compare_digits(ctx, op, x, digits, digit_bits, threshold, out_level=1, result_type=LOGIC):
+    """x op threshold (CMP_*) for B radix-encrypted integers x (B x digits x
+    2049 lvl2 words, as encrypt_digits) against a public threshold: one lvl2
+    bootstrap per digit plus one combining bootstrap (fhe_compare_digits_dev).
+    Returns lvl1 (out_level 1) or lvl2 TLWEs."""
+    x = np.ascontiguousarray(x, np.uint64)
+    B = x.shape[0]
+    if x.shape != (B, digits, LVL2_N + 1):
+        raise ValueError(f"x must be B x {digits} x {LVL2_N + 1} lvl2 words, got {x.shape}")
+    dx = DeviceBuffer.from_array(x) if x.size else DeviceBuffer(8)
+    N, dt = (LVL1_N, np.uint32) if out_level == 1 else (LVL2_N, np.uint64)
+    dout = DeviceBuffer(max(B, 1) * (N + 1) * np.dtype(dt).itemsize)
+    check(lib().fhe_compare_digits_dev(ctx.h, op, dx.ptr, digits, digit_bits, int(threshold),
+                                       out_level, dout.ptr, result_type, B, None))
+    ctx.sync()
+    return dout.to_array(dt, (B, N + 1))
+
+
+def digit_testvectors(op, digits, digit_bits, threshold, result_type=LOGIC):
+    """The test vectors of compare_digits built on the host:
+    ((digits + 1) x 2048 words, combine body offset, result body offset)."""
+    tv = np.zeros((digits + 1, LVL2_N), np.uint64)
+    add, post = ctypes.c_uint64(), ctypes.c_uint64()
+    check(lib().fhe_digit_testvectors(op, digits, digit_bits, int(threshold), result_type,
+                                      _ptr(tv), ctypes.byref(add), ctypes.byref(post)))
+    return tv, add.value, post.value
 
 
 def hom_not(ctx, level, a):

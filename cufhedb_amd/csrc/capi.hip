@@ -111,6 +111,8 @@ int fhe_ctx_destroy(fhe_ctx *ctx)
         if (b) (void)hipFree(b);
     for (uint32_t *b : ctx->chain)
         if (b) (void)hipFree(b);
+    for (uint32_t *b : ctx->digit)
+        if (b) (void)hipFree(b);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

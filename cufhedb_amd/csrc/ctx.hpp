@@ -40,6 +40,11 @@ struct fhe_ctx {
     static constexpr int kChainBufs = 7;
     uint32_t *chain[kChainBufs] = {};
     size_t chain_cap[kChainBufs] = {};
+    // scratch of the digit-wise comparison: lvl0 digits, bootstrapped
+    // digits, their combination, test vectors
+    static constexpr int kDigitBufs = 4;
+    uint32_t *digit[kDigitBufs] = {};
+    size_t digit_cap[kDigitBufs] = {};
 };
 
 namespace fhe {

@@ -236,7 +236,8 @@ constexpr int kQRow = 4 * 4 * 64;  // double2 per (i, row, column): [wave][q][la
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_blind_rotate2q(const uint32_t *__restrict__ in, int n0,
                                                          const cd *__restrict__ bk,
                                                          const uint64_t *__restrict__ tv,
-                                                         int tv_stride, uint64_t *__restrict__ out,
+                                                         int tv_stride, int tv_period,
+                                                         uint64_t *__restrict__ out,
                                                          int sample_extract, int B)
 {
     __shared__ uint64_t acc[2 * N2q];
@@ -246,7 +247,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const int ct = blockIdx.x;
     if (ct >= B) return;
     const uint32_t *c0 = in + size_t(ct) * (n0 + 1);
-    const uint64_t *tvp = tv + size_t(ct) * tv_stride;
+    const uint64_t *tvp = tv + size_t(tv_period > 0 ? ct % tv_period : ct) * tv_stride;
     const TwQ tw = make_twq(lane, wave);
 
     const uint32_t bbar = 2u * N2q - (c0[n0] >> (32 - 1 - Lvl2::nbit));
@@ -383,11 +384,11 @@ hipError_t launch_bk2q_from_int(const uint64_t *bk_int, cd *bk_dev, int n0, hipS
 
 hipError_t launch_blind_rotate2q(const uint32_t *in, int n0, const cd *bk, const uint64_t *tv,
                                  int tv_stride, uint64_t *out, bool sample_extract, int B,
-                                 hipStream_t s)
+                                 hipStream_t s, int tv_period)
 {
     if (B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_blind_rotate2q, dim3(B), dim3(256), 0, s, in, n0, bk, tv, tv_stride, out,
-                       int(sample_extract), B);
+    hipLaunchKernelGGL(k_blind_rotate2q, dim3(B), dim3(256), 0, s, in, n0, bk, tv, tv_stride,
+                       tv_period, out, int(sample_extract), B);
     return hipGetLastError();
 }
 

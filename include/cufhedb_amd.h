@@ -270,6 +270,29 @@ int fhe_compare_dev(fhe_ctx *ctx, int level, int op, const void *a_dev, const vo
  * every word of B TLWEs of level 0, 1 or 2 (out may equal in). */
 int fhe_hom_not_dev(fhe_ctx *ctx, int level, const void *in_dev, void *out_dev, int B,
                     void *stream);
+/* Digit-wise comparison with a public threshold: x op threshold for B
+ * integers given as radix-2^digit_bits digits.  No single reference entry
+ * point: it replaces HomCompare_gpu.cuh:14-76 (a op b through HomMSB chains,
+ * b an encrypted or trivially encrypted threshold) for the case of a
+ * threshold known in the clear.
+ *   x_dev : B x digits lvl2 TLWEs (2049 words each), row-major by value,
+ *           digit i (least significant first) of value b at
+ *           x_dev + (b * digits + i) * 2049, holding its digit v at
+ *           v * 2^(63 - digit_bits) (one padding bit);
+ *   digits in [1, 5], digit_bits in [1, 4], threshold < 2^(digits * digit_bits);
+ *   out_level 2: B lvl2 TLWEs, 1: B lvl1 TLWEs (through IKS lvl21);
+ *   result_type LOGIC (+-mu) or ARITHMETIC (0 / 2mu) as fhe_compare_dev.
+ * One lvl02 bootstrap per digit and one combining bootstrap per value.
+ * Needs BK lvl02 and KSK 20, and KSK 21 for out_level 1. */
+int fhe_compare_digits_dev(fhe_ctx *ctx, int op, const uint64_t *x_dev, int digits, int digit_bits,
+                           uint64_t threshold, int out_level, void *out_dev, int result_type, int B,
+                           void *stream);
+/* The test vectors fhe_compare_digits_dev builds on the device, on the host:
+ * tv receives (digits + 1) x 2048 words (one per digit position, then the
+ * combining sign bootstrap's); *combine_add is the body offset added to the
+ * sum of the bootstrapped digits, *post_add the one added to the result. */
+int fhe_digit_testvectors(int op, int digits, int digit_bits, uint64_t threshold, int result_type,
+                          uint64_t *tv, uint64_t *combine_add, uint64_t *post_add);
 
 /* ---- coalescing scheduler (mixed lvl1 / lvl2 request stream) ------------ */
 /* Replaces the reference's serving pattern for a stream of independent

@@ -276,3 +276,73 @@ def tlwe_phase64(c, key):
     n = key.shape[0]
     dot = (c[:, :n] * np.asarray(key, np.uint64)[None, :]).sum(axis=1, dtype=np.uint64)
     return c[:, n] - dot
+
+
+# ------------------------------------------- digit-wise comparison (lvl2)
+# No reference counterpart: the operator replaces the HomMSB-chain compare
+# (src/cuHEDB/HomCompare_gpu.cuh:14-76) for a threshold known in the clear.
+# This restatement is pinned by the mathematics: decisions against Python's
+# own comparison operators over every (x, t) pair of small radixes
+# (test_digits.py) and over encrypted rows (test_gpu_digits.py).
+CMP_GT, CMP_GE, CMP_LT, CMP_LE, CMP_EQ = 0, 1, 2, 3, 4
+
+
+def digit_terms(op, digits, bits, t):
+    """terms[i][v]: the integer digit position i contributes for digit value
+    v when the threshold is t -- the lexicographic comparison unrolled: the
+    least significant digit as a 0/1 comparison, each higher digit as a
+    sign weighted by 2^(i-1), which exceeds everything below it; for
+    equality, 1 per differing digit."""
+    R = 1 << bits
+    terms = []
+    for i in range(digits):
+        ti = (t >> (i * bits)) % R
+        row = []
+        for v in range(R):
+            if op == CMP_EQ:
+                row.append(int(v != ti))
+            elif i == 0:
+                row.append(int(v > ti) if op in (CMP_GT, CMP_LE) else int(v >= ti))
+            else:
+                row.append(((v > ti) - (v < ti)) << (i - 1))
+        terms.append(row)
+    return terms
+
+
+def digit_testvectors(op, digits, bits, t, result_type=1):
+    """(digits + 1) x 2048 test vectors: digit v's box is the coefficients
+    j whose phase j / 4096 rounds to v / 2^(bits+1); the box that rounds to
+    1/2 belongs to digit 0 approached from below zero and holds the negated
+    value (negacyclic wrap).  Row `digits` is the sign bootstrap's constant."""
+    N = 2048
+    R = 1 << bits
+    j = np.arange(N)
+    v = (j * 2 * R + N) // (2 * N)          # floor(j / box + 1/2), box = N / R
+    terms = digit_terms(op, digits, bits, t)
+    tv = np.zeros((digits + 1, N), np.uint64)
+    for i in range(digits):
+        f = np.array([terms[i][x % R] for x in v], dtype=np.int64)
+        f = np.where(v == R, -f, f)
+        tv[i] = (f.astype(np.uint64) << np.uint64(64 - digits))
+    mu = 1 << 61
+    u = mu if result_type else mu << 1
+    tv[digits] = u if op in (CMP_GT, CMP_GE) else (1 << 64) - u
+    combine_add = (1 << 64) - (1 << (63 - digits))
+    post_add = 0 if result_type else u
+    return tv, combine_add, post_add
+
+
+def compare_digits(x, op, digits, bits, t, ksk20, bkfft2, n0, result_type=1, ksk21=None):
+    """x: B x digits lvl2 TLWEs -> B lvl2 (ksk21 None) or lvl1 TLWEs, through
+    the oracle's key switch and bootstrap."""
+    tv, add, post = digit_testvectors(op, digits, bits, t, result_type)
+    out = []
+    for row in np.asarray(x, np.uint64):
+        acc = np.zeros(2049, np.uint64)
+        for i in range(digits):
+            acc += pbs64(iks64(row[i], ksk20, n0, 2, 10), bkfft2, tv[i])
+        acc[2048:] += np.uint64(add)
+        res = pbs64(iks64(acc, ksk20, n0, 2, 10), bkfft2, tv[digits])
+        res[2048:] += np.uint64(post)
+        out.append(res if ksk21 is None else iks64(res, ksk21, 1024, 10, 3))
+    return np.array(out)
