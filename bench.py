@@ -109,12 +109,14 @@ def cpu_baseline(threads, seconds_budget=12.0):
     }
 
 
-def operator_context(F):
-    """A context with every key the operator layer uses, on the reference's
-    own parameter set (n0 = 672, alpha0 = 2^-16).  The lvl2 comparison chains
-    need it: with n0 = 636 / alpha0 = 2^-15 the key-switched lvl0 noise sits
-    about 3 sigma from the 6-bit windows' decision boundaries."""
-    p = F.preset(F.PRESET_REF672)
+def operator_context(F, which=None):
+    """A context with every key the operator layer uses, by default on the
+    reference's own parameter set (n0 = 672, alpha0 = 2^-16).  The lvl2
+    comparison chains need it: with n0 = 636 / alpha0 = 2^-15 the
+    key-switched lvl0 noise sits about 3 sigma from the 6-bit windows'
+    decision boundaries (the digit-wise comparison keeps 2^-6 of margin and
+    also runs at n0 = 636)."""
+    p = F.preset(F.PRESET_REF672 if which is None else which)
     sk = F.SecretKeys(p, 20261212)
     ctx = F.Context(p, 0)
     ctx.load_bk(1, F.keygen_bk_lvl01(sk, 20261213), F.BK_INT)
@@ -124,27 +126,59 @@ def operator_context(F):
     return ctx, sk, p
 
 
+def timed_on(F, ctx, fn, reps):
+    """Mean milliseconds of fn over `reps` runs after one untimed run (which
+    also sizes the operator scratch), HIP events on the context stream."""
+    import ctypes
+
+    L_ = F.lib()
+    fn()
+    ctx.sync()
+    F.check(L_.fhe_timer_start(ctx.h, None))
+    for _ in range(reps):
+        fn()
+    ms = ctypes.c_float()
+    F.check(L_.fhe_timer_stop(ctx.h, None, ctypes.byref(ms)))
+    return float(ms.value) / reps
+
+
+def run_compare_digits(F, ctx, sk, p, rows, reps):
+    """x <= t for `rows` seeded uniform 20-bit values against a seeded public
+    threshold, digit-wise: five 4-bit lvl2 digits per value, one lvl2 PBS per
+    digit plus one combining PBS, lvl1 results (IKS lvl21)."""
+    rng = np.random.default_rng(19)
+    x = rng.integers(0, 1 << 20, rows).astype(np.uint64)
+    t = int(rng.integers(0, 1 << 20))
+    dx = F.DeviceBuffer.from_array(F.encrypt_digits(321, 0, sk.s2, x, 5, 4, p.alpha2))
+    out = F.DeviceBuffer(rows * (N + 1) * 4)
+    L_ = F.lib()
+    ms = timed_on(F, ctx, lambda: F.check(L_.fhe_compare_digits_dev(
+        ctx.h, F.CMP_LE, dx.ptr, 5, 4, t, 1, out.ptr, F.LOGIC, rows, None)), reps)
+    ph = F.tlwe_phase_u32(out.to_array(np.uint32, (rows, N + 1)), sk.s1).view(np.int32)
+    errors = int(np.sum((ph > 0) != (x <= t)))
+    dx.free()
+    out.free()
+    print(f"bench: digit-wise compare {rows} rows, n0={p.n0}: {ms:.1f} ms/query, {errors} errors",
+          file=sys.stderr, flush=True)
+    return {"workload": f"x <= t over {rows} rows of 20-bit values as five 4-bit lvl2 digits, "
+                        f"public threshold, n0={p.n0} (6 lvl2 PBS per row: one per digit + "
+                        "one combining; lvl1 result)",
+            "rows": rows, "ms_per_query": ms, "errors": errors}
+
+
 def run_queries(F, ctx, sk, p, rows, reps):
     """Encrypted query pipelines, each timed with HIP events on the context
     stream over `reps` runs after one untimed run (which also sizes the
     operator scratch); inputs resident on the device, results decrypted and
     compared with the plaintext query."""
     from cufhedb_amd import queries as Q
-    import ctypes
 
     L_ = F.lib()
     rng = np.random.default_rng(13)
     res = {"preset": f"n0={p.n0} (reference parameter set)"}
 
     def timed(fn):
-        fn()
-        ctx.sync()
-        F.check(L_.fhe_timer_start(ctx.h, None))
-        for _ in range(reps):
-            fn()
-        ms = ctypes.c_float()
-        F.check(L_.fhe_timer_stop(ctx.h, None, ctypes.byref(ms)))
-        return float(ms.value) / reps
+        return timed_on(F, ctx, fn, reps)
 
     # lvl2 PBS throughput (the chains' dominant operation): batch 4096
     B2 = 4096
@@ -182,6 +216,7 @@ def run_queries(F, ctx, sk, p, rows, reps):
           file=sys.stderr, flush=True)
     for b in (dx, dt, out):
         b.free()
+    res["compare_digits"] = run_compare_digits(F, ctx, sk, p, rows, reps)
 
     # TPC-H Q1 filter: shipdate <= date, 4 (returnflag, linestatus) groups
     date_bits = 9
@@ -374,6 +409,12 @@ def main():
             res["mixed_stream"] = {"sequential": seq, "coalesced": coal}
             print(f"bench: mixed stream: {res['mixed_stream']}", file=sys.stderr, flush=True)
         octx.close()
+        if not args.no_query:
+            # the 20-bit comparison workload at its own parameter set, n0 = 636
+            octx, osk, op = operator_context(F, F.PRESET_CFG636)
+            res["queries"]["compare_digits_n636"] = run_compare_digits(
+                F, octx, osk, op, args.query_rows, args.query_reps)
+            octx.close()
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(threads)

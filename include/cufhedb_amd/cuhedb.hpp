@@ -11,7 +11,9 @@
 //   cuHEDB:: comparison layer        src/cuHEDB/msb_gpu.cuh:7-193,
 //                                    src/cuHEDB/HomCompare_gpu.cuh:14-76
 //     HomMSB, greater_than, greater_than_equal, less_than, less_than_equal,
-//     equal (+ *_batch forms with an optional broadcast second operand)
+//     equal (+ *_batch forms with an optional broadcast second operand);
+//     compare_digits_batch, the same comparisons against a public threshold
+//     on radix-encrypted values (no reference counterpart)
 // P is any parameter struct with TFHEpp's member names (n, k, l, Bgbit, T):
 // TFHEpp::lvl1param / lvl2param work unchanged, and cufhedb_amd::lvl1param /
 // lvl2param below are provided for code that does not include TFHEpp.
@@ -453,6 +455,27 @@ CUFHEDB_AMD_CMP(less_than, FHE_CMP_LT)
 CUFHEDB_AMD_CMP(less_than_equal, FHE_CMP_LE)
 CUFHEDB_AMD_CMP(equal, FHE_CMP_EQ)
 #undef CUFHEDB_AMD_CMP
+
+// x op threshold for B integers given as radix-2^digit_bits digits, each a
+// lvl2 TLWE (x: B x digits x 2049 words, least significant digit first),
+// threshold public: one lvl2 bootstrap per digit plus one combining
+// bootstrap (fhe_compare_digits_dev; no reference counterpart -- the
+// public-threshold form of greater_than .. equal).  res: B lvl1 TLWEs.
+inline void compare_digits_batch(cufhedb_amd::Runtime &rt, int op, const uint64_t *x, int digits,
+                                 int digit_bits, uint64_t threshold, uint32_t *res,
+                                 bool result_type, int B)
+{
+    const size_t in_bytes = size_t(B) * digits * (cufhedb_amd::lvl2param::n + 1) * 8;
+    const size_t out_bytes = size_t(B) * (cufhedb_amd::lvl1param::n + 1) * 4;
+    cufhedb_amd::DeviceBuffer dx(in_bytes), dout(out_bytes);
+    dx.upload(x, in_bytes);
+    cufhedb_amd::check(fhe_compare_digits_dev(rt.ctx(), op, dx.as<uint64_t>(), digits, digit_bits,
+                                              threshold, 1, dout.get(), result_type ? 1 : 0, B,
+                                              nullptr),
+                       "fhe_compare_digits_dev");
+    cufhedb_amd::check(fhe_ctx_sync(rt.ctx()), "fhe_ctx_sync");
+    dout.download(res, out_bytes);
+}
 
 // LOG_to_ARI / log_rescale (gatebootstrap_gpu.cuh:155-170, HEDB CPU test vector)
 inline void LOG_to_ARI(cufhedb_amd::Runtime &rt, TLWELvl1 &res, const TLWELvl1 &tlwe,

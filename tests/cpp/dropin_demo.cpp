@@ -146,6 +146,30 @@ int main()
         cuHEDB::equal(rt, cx, cy, r, bits, cuHEDB::LOGIC, 0);
         bad += (int32_t(phase(r.data(), s1.data(), 1024)) > 0) != (x == y);
     }
+    // cuHEDB::compare_digits_batch: 12-bit values as three 4-bit lvl2
+    // digits against a public threshold (lvl02 key and KSK 20 / 21 made here)
+    {
+        std::vector<uint64_t> bk2(size_t(n0) * 8 * 2 * 2048);
+        cufhedb_amd::check(fhe_keygen_bk_lvl02(&p, 16, s0.data(), s2.data(), bk2.data()), "bk2");
+        rt.load_bk(2, bk2.data(), FHE_BK_INT);
+        cufhedb_amd::check(fhe_keygen_ksk_dev(rt.ctx(), 20, 17, s2.data(), s0.data(), nullptr),
+                           "ksk20");
+        cufhedb_amd::check(fhe_keygen_ksk_dev(rt.ctx(), 21, 18, s2.data(), s1.data(), nullptr),
+                           "ksk21");
+        const int B = 24, D = 3;
+        const uint64_t t = 0x5A7;
+        std::vector<uint64_t> vals(B), m(size_t(B) * D), x(size_t(B) * D * 2049);
+        for (int i = 0; i < B; i++) {
+            vals[i] = i < 4 ? t + i - 2 : gen() % 4096;
+            for (int d = 0; d < D; d++) m[size_t(i) * D + d] = ((vals[i] >> (4 * d)) & 15) << 59;
+        }
+        cufhedb_amd::check(fhe_tlwe_encrypt_u64(19, 0, s2.data(), 2048, m.data(), p.alpha2,
+                                                x.data(), B * D), "enc2");
+        std::vector<uint32_t> r(size_t(B) * 1025);
+        cuHEDB::compare_digits_batch(rt, FHE_CMP_LE, x.data(), D, 4, t, r.data(), cuHEDB::LOGIC, B);
+        for (int i = 0; i < B; i++)
+            bad += (int32_t(phase(r.data() + size_t(i) * 1025, s1.data(), 1024)) > 0) != (vals[i] <= t);
+    }
     std::printf("dropin_demo: %s (%d mismatches)\n", bad ? "FAILED" : "ok", bad);
     return bad ? 1 : 0;
 }
