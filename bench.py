@@ -335,11 +335,16 @@ def main():
         ph = F.tlwe_phase_u32(out, sk.s1).view(np.int32).astype(np.int64)
         dec = np.round(ph / (1 << 28)).astype(np.int64) % 16
         err = ph - lut[m] * (1 << 28)
+        # |phase error| per octave, in 2^-32 torus units (decision margin 2^27)
+        edges = [0] + [1 << j for j in range(18, 28)] + [1 << 62]
+        hist = np.histogram(np.abs(err), bins=edges)[0]
+        labels = ["<2^18"] + [f"2^{j}" for j in range(18, 27)] + [">=2^27"]
         variants[name] = {
             "ms_per_step": total_ms / args.steps,
             "pbs_per_s": B * args.steps / (total_ms / 1e3),
             "decrypt_failures": int(np.sum(dec != lut[m])),
             "phase_error_std_log2": float(np.log2(np.std(err))),
+            "phase_error_hist": {lab: int(c) for lab, c in zip(labels, hist)},
         }
         print(f"bench: {name}: timed {args.steps} steps, {total_ms / args.steps:.3f} ms/step, "
               f"{variants[name]['decrypt_failures']} decryption failures", file=sys.stderr,

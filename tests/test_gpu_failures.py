@@ -37,6 +37,8 @@ def test_no_decryption_failures_in_1e6_pbs(keys636, ctx636):
     L = F.lib()
     failures, total = 0, 0
     err_max, err_sq = 0, 0.0
+    edges = [0] + [1 << j for j in range(18, 28)] + [1 << 62]
+    hist = np.zeros(len(edges) - 1, np.int64)
     for b in range(batches):
         m = rng.integers(0, 8, B)
         lut = rng.integers(0, 8, 8)
@@ -56,8 +58,12 @@ def test_no_decryption_failures_in_1e6_pbs(keys636, ctx636):
         total += B
         err_max = max(err_max, int(np.max(np.abs(err))))
         err_sq += float(np.sum(err.astype(np.float64) ** 2))
+        hist += np.histogram(np.abs(err), bins=edges)[0]
     std = (err_sq / total) ** 0.5
+    labels = ["<2^18"] + [f"2^{j}" for j in range(18, 27)] + [">=2^27"]
     print(f"failures {failures} of {total} PBS; phase error std 2^{np.log2(std):.2f}, "
-          f"max |err| 2^{np.log2(max(err_max, 1)):.2f} (decision margin 2^27)")
+          f"max |err| 2^{np.log2(max(err_max, 1)):.2f} (decision margin 2^27); |err| per "
+          f"octave (2^-32 units): {dict(zip(labels, (int(c) for c in hist)))}")
+    assert hist[-1] == failures
     assert total >= 1_000_000
     assert failures == 0
