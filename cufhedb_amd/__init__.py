@@ -123,6 +123,10 @@ _SIGS = {
     "fhe_compare_dev": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _u32, _i, _i, _vp]),
     "fhe_hom_not_dev": (_i, [_vp, _i, _vp, _vp, _i, _vp]),
     "fhe_compare_digits_dev": (_i, [_vp, _i, _vp, _i, _i, _u64, _i, _vp, _i, _i, _vp]),
+    "fhe_graph_begin": (_i, [_vp]),
+    "fhe_graph_end": (_i, [_vp, _vp]),
+    "fhe_graph_launch": (_i, [_vp, _vp]),
+    "fhe_graph_destroy": (_i, [_vp]),
     "fhe_digit_testvectors": (_i, [_i, _i, _i, _u64, _i, _vp, _vp, _vp]),
     "fhe_keyfile_params_of": (_i, [_vp, _vp]),
     "fhe_keyfile_payload_bytes": (_u64, [_i, _u32]),
@@ -608,6 +612,41 @@ def compare(ctx, level, op, a, b, plain_bits, result_type=LOGIC):
                                 result_type, B, None))
     ctx.sync()
     return dout.to_array(np.uint32, (B, LVL1_N + 1))
+
+
+class Graph:
+    """A pipeline of _dev calls on ``ctx``'s stream captured into a HIP graph
+    (fhe_graph_*).  ``fn()`` is called once while capturing, so its calls are
+    recorded, not run; ``launch()`` replays them on the same device buffers.
+    Run ``fn`` once eagerly at the same sizes first (scratch cannot be
+    allocated during a capture)."""
+
+    def __init__(self, ctx, fn):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        check(lib().fhe_graph_begin(ctx.h))
+        try:
+            fn()
+        except BaseException:
+            g = ctypes.c_void_p()
+            if lib().fhe_graph_end(ctx.h, ctypes.byref(g)) == 0:
+                lib().fhe_graph_destroy(g)
+            raise
+        check(lib().fhe_graph_end(ctx.h, ctypes.byref(self.h)))
+
+    def launch(self):
+        check(lib().fhe_graph_launch(self.h, None))
+
+    def destroy(self):
+        if self.h:
+            lib().fhe_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def encrypt_digits(seed, index0, s2, values, digits, digit_bits, alpha):

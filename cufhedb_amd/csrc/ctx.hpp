@@ -68,9 +68,18 @@ inline hipStream_t pick(fhe_ctx *ctx, void *stream)
     return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
 }
 
+// set on this thread between fhe_graph_begin and fhe_graph_end: scratch
+// must not be (re)allocated while the context stream is being captured
+extern thread_local bool g_capturing;
+
 inline int grow(uint32_t *&buf, size_t &cap, size_t words)
 {
     if (cap >= words) return FHE_OK;
+    if (g_capturing) {
+        set_error("scratch of %zu words needed during graph capture: run the pipeline once "
+                  "at this size before capturing it", words);
+        return FHE_ERR_ARG;
+    }
     if (buf) (void)hipFree(buf);
     buf = nullptr;
     cap = 0;

@@ -294,6 +294,22 @@ int fhe_compare_digits_dev(fhe_ctx *ctx, int op, const uint64_t *x_dev, int digi
 int fhe_digit_testvectors(int op, int digits, int digit_bits, uint64_t threshold, int result_type,
                           uint64_t *tv, uint64_t *combine_add, uint64_t *post_add);
 
+/* ---- HIP graph capture of pipelines ------------------------------------ */
+/* Replaces the reference's one-host-call-and-synchronise-per-bootstrap
+ * driving of a query (cufft_gpu.cuh:262, TPCH_Q1.cu:255-283): the _dev calls
+ * made on the context stream (stream argument NULL) between fhe_graph_begin
+ * and fhe_graph_end are recorded, not run, and fhe_graph_launch replays them
+ * as one graph on the buffers they were given.  Run the pipeline once at the
+ * same sizes before capturing it: scratch that would have to grow during a
+ * capture makes the call fail with FHE_ERR_ARG (end the capture with
+ * fhe_graph_end all the same).  Host-pointer forms and key loading cannot be
+ * captured.  One capture per thread at a time. */
+typedef struct fhe_graph fhe_graph;
+int fhe_graph_begin(fhe_ctx *ctx);
+int fhe_graph_end(fhe_ctx *ctx, fhe_graph **out);
+int fhe_graph_launch(fhe_graph *g, void *stream /* NULL = the context stream */);
+int fhe_graph_destroy(fhe_graph *g);
+
 /* ---- coalescing scheduler (mixed lvl1 / lvl2 request stream) ------------ */
 /* Replaces the reference's serving pattern for a stream of independent
  * requests -- one host thread per CUDA stream, one ciphertext per call

@@ -28,6 +28,8 @@
 //   * Batched forms (the *_batch members) take B ciphertexts per call; that
 //     is the fast path.
 //   * Errors throw cufhedb_amd::Error (the reference ignores CUDA errors).
+//   cufhedb_amd::Graph                a captured pipeline of _dev calls,
+//                                     replayed as one HIP graph launch
 //   cufhedb_amd::Scheduler            the per-stream_id serving loop of the
 //                                     query programs (TPCH_Q6.cu:179-193),
 //                                     as merged batched launches
@@ -161,6 +163,34 @@ class Scheduler {
 
   private:
     fhe_sched *s_ = nullptr;
+};
+
+// A pipeline of _dev calls on the Runtime's stream captured into a HIP graph
+// (fhe_graph_*): the callable runs once under capture (its calls are
+// recorded, not executed); launch() replays the recording.  Run the pipeline
+// once eagerly at the same sizes before capturing it.
+class Graph {
+  public:
+    template <class Fn>
+    Graph(Runtime &rt, Fn &&fn)
+    {
+        check(fhe_graph_begin(rt.ctx()), "fhe_graph_begin");
+        try {
+            fn();
+        } catch (...) {
+            fhe_graph *g = nullptr;
+            if (fhe_graph_end(rt.ctx(), &g) == FHE_OK) fhe_graph_destroy(g);
+            throw;
+        }
+        check(fhe_graph_end(rt.ctx(), &g_), "fhe_graph_end");
+    }
+    ~Graph() { fhe_graph_destroy(g_); }
+    Graph(const Graph &) = delete;
+    Graph &operator=(const Graph &) = delete;
+    void launch(void *stream = nullptr) { check(fhe_graph_launch(g_, stream), "fhe_graph_launch"); }
+
+  private:
+    fhe_graph *g_ = nullptr;
 };
 
 }  // namespace cufhedb_amd
