@@ -126,6 +126,20 @@ def operator_context(F, which=None):
     return ctx, sk, p
 
 
+TUNE_BATCHES = (32, 128, 512, 1024, 2048, 4096, 16384)
+
+
+def autotune(F, ctx):
+    """Tile of the lvl2 -> lvl1 key switch per batch size, measured on this
+    GPU and kept in the context for the query pipelines."""
+    res = ctx.autotune(TUNE_BATCHES)
+    print(f"bench: autotuned lvl21 key-switch tiles {ctx.tune_table()}", file=sys.stderr,
+          flush=True)
+    return {"knob": "ciphertexts per workgroup of the lvl2 -> lvl1 key switch",
+            "table": {str(b): t for b, (t, _) in res.items()},
+            "ms": {str(b): ms for b, (_, ms) in res.items()}}
+
+
 def timed_on(F, ctx, fn, reps):
     """Mean milliseconds of fn over `reps` runs after one untimed run (which
     also sizes the operator scratch), HIP events on the context stream."""
@@ -242,13 +256,34 @@ def run_queries(F, ctx, sk, p, rows, reps):
         want = ok & (rf == a) & (ls == b)
         errors += int(np.sum(pred[g] != want))
         errors += int(int(np.sum(qty * pred[g])) != int(np.sum(qty[want])))
+    # the same pipeline captured once into a HIP graph and replayed
+    graph = F.Graph(ctx, lambda: Q.tpch_q1_filter(ctx, cols[0], cols[1], cols[2], dthr, one,
+                                                  rows, scratch=scratch))
+    ms_graph = timed(graph.launch)
+    pred = Q.decrypt_predicates(scratch.out, rows, sk.s1)
+    errors_graph = sum(int(np.sum(pred[g] != (ok & (rf == a) & (ls == b))))
+                       for g, (a, b) in enumerate(((0, 0), (0, 1), (1, 0), (1, 1))))
+    graph.destroy()
+    # small batches, where host launch overhead shows: eager calls vs graph
+    small = 256
+    sscr = Q.Q1Scratch(small)
+    run_small = lambda: Q.tpch_q1_filter(ctx, cols[0], cols[1], cols[2], dthr, one, small,  # noqa
+                                         scratch=sscr)
+    ms_small = timed_on(F, ctx, run_small, 20)
+    gsmall = F.Graph(ctx, run_small)
+    ms_small_graph = timed_on(F, ctx, gsmall.launch, 20)
+    gsmall.destroy()
+    sscr.free()
     res["tpch_q1_filter"] = {
         "workload": f"TPC-H Q1 filter on {rows} lvl1 rows: shipdate <= date (9 bit), "
                     "4 (returnflag, linestatus) groups, log_rescale; SUM(qty)/COUNT after "
                     "decryption",
-        "rows": rows, "ms_per_query": ms, "errors": errors}
-    print(f"bench: TPC-H Q1 filter {rows} rows: {ms:.1f} ms/query, {errors} errors",
-          file=sys.stderr, flush=True)
+        "rows": rows, "ms_per_query": ms, "errors": errors,
+        "ms_per_query_graph": ms_graph, "errors_graph": errors_graph,
+        "small_rows": small, "small_ms_eager": ms_small, "small_ms_graph": ms_small_graph}
+    print(f"bench: TPC-H Q1 filter {rows} rows: {ms:.1f} ms/query, {errors} errors; "
+          f"graph replay {ms_graph:.1f} ms, {errors_graph} errors; {small} rows: eager "
+          f"{ms_small:.2f} ms, graph {ms_small_graph:.2f} ms", file=sys.stderr, flush=True)
     scratch.free()
     for b in cols + [dthr, one]:
         b.free()
@@ -404,6 +439,7 @@ def main():
     res["queries"] = None
     if not args.no_query or args.mixed:
         octx, osk, op = operator_context(F)
+        res["autotune"] = autotune(F, octx)
         if not args.no_query:
             res["queries"] = run_queries(F, octx, osk, op, args.query_rows, args.query_reps)
         if args.mixed:
@@ -417,6 +453,7 @@ def main():
         if not args.no_query:
             # the 20-bit comparison workload at its own parameter set, n0 = 636
             octx, osk, op = operator_context(F, F.PRESET_CFG636)
+            octx.set_tune_table(sorted((int(b), t) for b, t in res["autotune"]["table"].items()))
             res["queries"]["compare_digits_n636"] = run_compare_digits(
                 F, octx, osk, op, args.query_rows, args.query_reps)
             octx.close()

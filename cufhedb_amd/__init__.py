@@ -123,6 +123,9 @@ _SIGS = {
     "fhe_compare_dev": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _u32, _i, _i, _vp]),
     "fhe_hom_not_dev": (_i, [_vp, _i, _vp, _vp, _i, _vp]),
     "fhe_compare_digits_dev": (_i, [_vp, _i, _vp, _i, _i, _u64, _i, _vp, _i, _i, _vp]),
+    "fhe_autotune": (_i, [_vp, _i, _vp, _i, _vp, _vp]),
+    "fhe_tune_set": (_i, [_vp, _i, _vp, _vp, _i]),
+    "fhe_tune_get": (_i, [_vp, _i, _vp, _vp, _i, _vp]),
     "fhe_graph_begin": (_i, [_vp]),
     "fhe_graph_end": (_i, [_vp, _vp]),
     "fhe_graph_launch": (_i, [_vp, _vp]),
@@ -409,6 +412,33 @@ class Context:
         stgt = sk.s1 if kind == 21 else sk.s0
         check(lib().fhe_keygen_ksk_dev(self.h, kind, seed, _ptr(sdom), _ptr(stgt),
                                        out_dev.ptr if out_dev is not None else None))
+
+    # ---- tile autotuning (fhe_autotune / fhe_tune_get / fhe_tune_set)
+    TUNE_CANDIDATES = (8, 16, 32)
+
+    def autotune(self, batches=(64, 256, 1024, 4096, 16384)):
+        """Time the lvl2 -> lvl1 key switch's tiles (8, 16, 32 ciphertexts
+        per workgroup) at each batch size and keep the fastest; returns
+        {batch: (tile, {tile: ms})}.  Needs KSK 21."""
+        b = np.ascontiguousarray(sorted(batches), np.int32)
+        choice = np.zeros(b.size, np.int32)
+        ms = np.zeros((b.size, 3), np.float32)
+        check(lib().fhe_autotune(self.h, 0, _ptr(b), int(b.size), _ptr(choice), _ptr(ms)))
+        return {int(x): (int(c), {t: float(v) for t, v in zip(self.TUNE_CANDIDATES, row)})
+                for x, c, row in zip(b, choice, ms)}
+
+    def tune_table(self):
+        """[(batch, tile), ...] of the lvl2 -> lvl1 key switch ([] = untuned)."""
+        b = np.zeros(16, np.int32)
+        t = np.zeros(16, np.int32)
+        n = ctypes.c_int()
+        check(lib().fhe_tune_get(self.h, 0, _ptr(b), _ptr(t), 16, ctypes.byref(n)))
+        return [(int(b[i]), int(t[i])) for i in range(n.value)]
+
+    def set_tune_table(self, table):
+        b = np.ascontiguousarray([x for x, _ in table], np.int32)
+        t = np.ascontiguousarray([y for _, y in table], np.int32)
+        check(lib().fhe_tune_set(self.h, 0, _ptr(b), _ptr(t), int(b.size)))
 
     def set_pbs_variant(self, variant):
         check(lib().fhe_set_pbs_variant(self.h, variant))

@@ -274,7 +274,7 @@ int fhe_iks_dev(fhe_ctx *ctx, int kind, const void *in_dev, uint32_t *out_dev, i
         return FHE_ERR_NOT_LOADED;
     }
     HIP_TRY(launch_iks(kind, in_dev, ctx->ksk[slot], out_dev, ks_target_n(ctx, kind), B,
-                       pick(ctx, stream)),
+                       pick(ctx, stream), iks_tile(ctx, kind, B)),
             "identity key switch");
     return FHE_OK;
 }

@@ -45,6 +45,12 @@ struct fhe_ctx {
     static constexpr int kDigitBufs = 4;
     uint32_t *digit[kDigitBufs] = {};
     size_t digit_cap[kDigitBufs] = {};
+    // tuned tile of the lvl2 -> lvl1 key switch per batch size (ascending
+    // batch sizes; a batch uses the entry of the largest size <= it)
+    static constexpr int kTuneMax = 16;
+    int tune21_batch[kTuneMax] = {};
+    int tune21_tile[kTuneMax] = {};
+    int tune21_n = 0;
 };
 
 namespace fhe {
@@ -100,6 +106,19 @@ inline int alloc_ksk(fhe_ctx *ctx, int kind)
     const size_t bytes = (fhe_ksk_words(&ctx->p, kind) + kKskPadWords) * sizeof(uint32_t);
     HIP_TRY(hipMalloc(&ctx->ksk[slot], bytes), "KSK allocation");
     return FHE_OK;
+}
+
+// workgroup tile of a key switch of B ciphertexts (only kind 21 has a
+// choice).  Untuned: 8 below 512 ciphertexts, 16 below 2048, else 32 (one
+// MI355X: 32 ciphertexts 1.9 / 2.7 / 9.2 ms with tiles 8 / 16 / 32, 2048
+// ciphertexts 19.1 / 17.3 / 13.6 ms)
+inline int iks_tile(const fhe_ctx *ctx, int kind, int B)
+{
+    if (kind != 21) return 0;
+    if (ctx->tune21_n == 0) return B < 512 ? 8 : B < 2048 ? 16 : 32;
+    int t = ctx->tune21_tile[0];
+    for (int i = 1; i < ctx->tune21_n && ctx->tune21_batch[i] <= B; i++) t = ctx->tune21_tile[i];
+    return t;
 }
 
 inline int ks_target_n(const fhe_ctx *ctx, int kind) { return kind == 21 ? Lvl1::N : int(ctx->p.n0); }

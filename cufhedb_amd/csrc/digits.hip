@@ -216,7 +216,7 @@ int fhe_compare_digits_dev(fhe_ctx *ctx, int op, const uint64_t *x_dev, int digi
     if (post) HIP_TRY(launch_tlwe_add_body<uint64_t>(res, W2, post, B, s), "result offset");
     if (out_level == 1)
         HIP_TRY(launch_iks(21, res, ctx->ksk[ksk_slot(21)], static_cast<uint32_t *>(out_dev),
-                           Lvl1::N, B, s),
+                           Lvl1::N, B, s, iks_tile(ctx, 21, B)),
                 "result key switch");
     return FHE_OK;
 }

@@ -371,8 +371,18 @@ hipError_t launch_tv_ide(T *tv, uint32_t scale_bits, int N, hipStream_t s)
 template hipError_t launch_tv_ide<uint32_t>(uint32_t *, uint32_t, int, hipStream_t);
 template hipError_t launch_tv_ide<uint64_t>(uint64_t *, uint32_t, int, hipStream_t);
 
+template <int C>
+void launch_iks21_tiled(const uint64_t *in, const uint32_t *ksk, uint32_t *out, int tn, int B,
+                        hipStream_t s)
+{
+    constexpr int TJ = KS21::t;
+    const dim3 grid(unsigned((B + C - 1) / C), unsigned((tn + 1 + 63) / 64));
+    hipLaunchKernelGGL((k_iks_tiled<uint64_t, Lvl2::N, KS21::t, KS21::basebit, C, TJ>), grid,
+                       dim3(256), 0, s, in, ksk, out, tn, B);
+}
+
 hipError_t launch_iks(int kind, const void *in, const uint32_t *ksk, uint32_t *out, int tn, int B,
-                      hipStream_t s)
+                      hipStream_t s, int tile)
 {
     if (B <= 0) return hipSuccess;
     switch (kind) {
@@ -385,13 +395,18 @@ hipError_t launch_iks(int kind, const void *in, const uint32_t *ksk, uint32_t *o
             static_cast<const uint64_t *>(in), ksk, out, tn, B, s);
         break;
     case 21: {
-        // 32 ciphertexts per tile, all 10 digit rows of a mask word in flight
+        // `tile` ciphertexts per workgroup (8, 16 or 32; 0 = 32), all 10 digit
+        // rows of a mask word in flight.  Larger tiles read each key row for
+        // more ciphertexts, smaller ones give small batches more workgroups
         // (64-ciphertext tiles measured 20-120 % slower: register and SGPR
         // pressure lower the occupancy more than the halved key traffic gains)
-        constexpr int C = 32, TJ = KS21::t;
-        const dim3 grid(unsigned((B + C - 1) / C), unsigned((tn + 1 + 63) / 64));
-        hipLaunchKernelGGL((k_iks_tiled<uint64_t, Lvl2::N, KS21::t, KS21::basebit, C, TJ>), grid,
-                           dim3(256), 0, s, static_cast<const uint64_t *>(in), ksk, out, tn, B);
+        const auto *x = static_cast<const uint64_t *>(in);
+        if (tile == 8)
+            launch_iks21_tiled<8>(x, ksk, out, tn, B, s);
+        else if (tile == 16)
+            launch_iks21_tiled<16>(x, ksk, out, tn, B, s);
+        else
+            launch_iks21_tiled<32>(x, ksk, out, tn, B, s);
         break;
     }
     default:

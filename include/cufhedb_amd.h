@@ -294,6 +294,23 @@ int fhe_compare_digits_dev(fhe_ctx *ctx, int op, const uint64_t *x_dev, int digi
 int fhe_digit_testvectors(int op, int digits, int digit_bits, uint64_t threshold, int result_type,
                           uint64_t *tv, uint64_t *combine_add, uint64_t *post_add);
 
+/* ---- tile autotuning ---------------------------------------------------- */
+/* No reference counterpart (the reference launches one ciphertext per call).
+ * FHE_TUNE_IKS21_TILE: ciphertexts per workgroup (8, 16 or 32) of the
+ * lvl2 -> lvl1 key switch -- more read each key row once for more
+ * ciphertexts, fewer give small batches more workgroups.  fhe_autotune
+ * times every candidate on the context's KSK 21 at the n ascending batch
+ * sizes and keeps the fastest per size (a batch then uses the entry of the
+ * largest size <= it; untuned: 8 below 512 ciphertexts, 16 below 2048,
+ * else 32); choice (n, may be NULL) receives the
+ * picks, ms (n x 3 in candidate order 8, 16, 32, may be NULL) the mean
+ * times.  All candidates compute identical results.  fhe_tune_get /
+ * fhe_tune_set read and restore the table (e.g. from a per-device cache). */
+enum fhe_tune_knob { FHE_TUNE_IKS21_TILE = 0 };
+int fhe_autotune(fhe_ctx *ctx, int knob, const int *batches, int n, int *choice, float *ms);
+int fhe_tune_set(fhe_ctx *ctx, int knob, const int *batches, const int *tiles, int n);
+int fhe_tune_get(fhe_ctx *ctx, int knob, int *batches, int *tiles, int cap, int *n);
+
 /* ---- HIP graph capture of pipelines ------------------------------------ */
 /* Replaces the reference's one-host-call-and-synchronise-per-bootstrap
  * driving of a query (cufft_gpu.cuh:262, TPCH_Q1.cu:255-283): the _dev calls

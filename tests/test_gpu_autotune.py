@@ -1,0 +1,71 @@
+"""Tile autotuning of the lvl2 -> lvl1 key switch (fhe_autotune, fhe_tune_*).
+
+Every tile (8, 16, 32 ciphertexts per workgroup) must give the oracle's
+key switch word for word (IdentityKeySwitch<lvl21param>,
+TFHEpp/src/keyswitch.cpp:12-48) at batch sizes that leave partial tiles;
+autotuning must leave results unchanged and record a valid table; bad
+tables are refused.
+"""
+
+import numpy as np
+import pytest
+
+import cufhedb_amd as F
+
+import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ksk21(full636):
+    k, _ = full636
+    # the context's KSK 21 was generated on the device from this seed; the
+    # host generator gives the same key word for word
+    return F.keygen_ksk(k.sk, 21, k.seed + 5)
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_every_tile_matches_oracle(full636, ksk21, tile):
+    k, ctx = full636
+    rng = np.random.default_rng(tile)
+    try:
+        ctx.set_tune_table([(1, tile)])
+        for B in (1, 7, 33, 100):
+            x = rng.integers(0, 2 ** 63, (B, F.LVL2_N + 1), dtype=np.uint64) * np.uint64(2) \
+                + rng.integers(0, 2, (B, F.LVL2_N + 1), dtype=np.uint64)
+            got = ctx.iks(21, x)
+            for b in sorted({0, B // 2, B - 1}):
+                assert np.array_equal(got[b], O.iks64(x[b], ksk21, 1024, 10, 3)), (B, b)
+    finally:
+        ctx.set_tune_table([])
+
+
+def test_autotune_keeps_results(full636):
+    k, ctx = full636
+    rng = np.random.default_rng(5)
+    x = rng.integers(0, 2 ** 63, (300, F.LVL2_N + 1), dtype=np.uint64) * np.uint64(2)
+    before = ctx.iks(21, x)
+    try:
+        res = ctx.autotune(batches=(32, 256, 2048))
+        print("\n  lvl21 key-switch tiles:", {b: (t, {c: round(v, 3) for c, v in ms.items()})
+                                             for b, (t, ms) in res.items()})
+        table = ctx.tune_table()
+        assert [b for b, _ in table] == [32, 256, 2048]
+        assert all(t in (8, 16, 32) for _, t in table)
+        assert all(all(v > 0 for v in ms.values()) for _, ms in res.values())
+        assert np.array_equal(ctx.iks(21, x), before)
+        assert np.array_equal(ctx.iks(21, x[:5]), before[:5])   # below the first size
+    finally:
+        ctx.set_tune_table([])
+    assert ctx.tune_table() == []
+
+
+def test_bad_tables_refused(full636):
+    _, ctx = full636
+    for table in ([(1, 12)], [(64, 8), (32, 16)], [(0, 8)], [(1, 8)] * 17):
+        with pytest.raises(F.FheError):
+            ctx.set_tune_table(table)
+    with pytest.raises(F.FheError):
+        ctx.autotune(batches=())
+    assert ctx.tune_table() == []
