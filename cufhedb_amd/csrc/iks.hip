@@ -6,10 +6,12 @@
 //   for i < dn, j < t:  a_ij = ((a_i + 2^(W-1-t*basebit)) >> (W-(j+1)basebit)) & mask
 //                       if a_ij != 0: res -= ksk[i][j][a_ij - 1]
 // The reference runs this on the host CPU before every bootstrap
-// (src/cuHEDB/gatebootstrap_gpu.cuh:42,59,...).  Here one workgroup switches
-// one ciphertext: the dn*t digits are computed once into LDS, then each
-// thread owns output columns and walks the selected key rows, so every row
-// read is a coalesced run of (tn+1) words.  The key stays in HBM in the
+// (src/cuHEDB/gatebootstrap_gpu.cuh:42,59,...).  Here a batch is one launch
+// of one of two kernels, chosen by the base: k_iks_rows (kinds 10 and 20,
+// basebit 10) reads every selected key row whole, one workgroup per
+// ciphertext; k_iks_tiled (kind 21, basebit 3) switches a tile of
+// ciphertexts on one 64-column slice so that the few candidate rows of a
+// digit position are read once per tile.  The key stays in HBM in the
 // reference's own layout [dn][t][2^basebit - 1][tn + 1].
 #include "launch.hpp"
 #include "params.hpp"
@@ -17,45 +19,6 @@
 namespace fhe {
 
 namespace {
-
-template <class TD, int DN, int T, int BB>
-__global__ __launch_bounds__(256) void k_iks(const TD *__restrict__ in,
-                                             const uint32_t *__restrict__ ksk,
-                                             uint32_t *__restrict__ out, int tn, int B)
-{
-    constexpr int W = int(sizeof(TD) * 8);
-    constexpr int NDIG = DN * T;
-    constexpr uint32_t mask = (1u << BB) - 1;
-    constexpr size_t nd = (size_t(1) << BB) - 1;
-    __shared__ uint16_t dig[NDIG];
-    const int b = blockIdx.x;
-    if (b >= B) return;
-    const TD *c = in + size_t(b) * (DN + 1);
-    const TD prec = TD(1) << (W - (1 + BB * T));
-    for (int e = threadIdx.x; e < NDIG; e += blockDim.x) {
-        const int i = e / T, j = e % T;
-        const TD abar = c[i] + prec;
-        dig[e] = uint16_t((abar >> (W - (j + 1) * BB)) & mask);
-    }
-    uint32_t body;
-    if constexpr (W == 32)
-        body = uint32_t(c[DN]);
-    else
-        body = uint32_t((c[DN] + (TD(1) << (W - 33))) >> (W - 32));
-    __syncthreads();
-    const size_t row = size_t(tn) + 1;
-    uint32_t *o = out + size_t(b) * row;
-    for (int k = threadIdx.x; k <= tn; k += blockDim.x) {
-        uint32_t acc = (k == tn) ? body : 0u;
-        const uint32_t *kc = ksk + k;
-#pragma unroll 8
-        for (int e = 0; e < NDIG; e++) {
-            const uint32_t d = dig[e];
-            if (d != 0) acc -= kc[(size_t(e) * nd + (d - 1)) * row];
-        }
-        o[k] = acc;
-    }
-}
 
 // Whole-row form for large bases (kinds 10 and 20: basebit 10, 1023
 // candidate rows per digit position, so ciphertexts rarely share a row): one
