@@ -126,18 +126,33 @@ def operator_context(F, which=None):
     return ctx, sk, p
 
 
-TUNE_BATCHES = (32, 128, 512, 1024, 2048, 4096, 16384)
+TUNE = {
+    "iks21_tile": ("ciphertexts per workgroup of the lvl2 -> lvl1 key switch",
+                   (32, 128, 512, 1024, 2048, 4096, 16384)),
+    "pbs1_waves": ("waves per ciphertext of the lvl1 blind rotation",
+                   (64, 256, 512, 1024, 2048)),
+}
 
 
 def autotune(F, ctx):
-    """Tile of the lvl2 -> lvl1 key switch per batch size, measured on this
-    GPU and kept in the context for the query pipelines."""
-    res = ctx.autotune(TUNE_BATCHES)
-    print(f"bench: autotuned lvl21 key-switch tiles {ctx.tune_table()}", file=sys.stderr,
-          flush=True)
-    return {"knob": "ciphertexts per workgroup of the lvl2 -> lvl1 key switch",
-            "table": {str(b): t for b, (t, _) in res.items()},
-            "ms": {str(b): ms for b, (_, ms) in res.items()}}
+    """Shapes measured on this GPU per batch size and kept in the context for
+    the query pipelines: the lvl2 -> lvl1 key switch's tile and the lvl1
+    blind rotation's waves per ciphertext."""
+    out = {}
+    for name, knob in (("iks21_tile", F.TUNE_IKS21_TILE), ("pbs1_waves", F.TUNE_PBS1_WAVES)):
+        what, batches = TUNE[name]
+        res = ctx.autotune(batches, knob=knob)
+        print(f"bench: autotuned {what}: {ctx.tune_table(knob=knob)}", file=sys.stderr,
+              flush=True)
+        out[name] = {"knob": what, "table": {str(b): t for b, (t, _) in res.items()},
+                     "ms": {str(b): ms for b, (_, ms) in res.items()}}
+    return out
+
+
+def apply_tuning(F, ctx, tuned):
+    for name, knob in (("iks21_tile", F.TUNE_IKS21_TILE), ("pbs1_waves", F.TUNE_PBS1_WAVES)):
+        ctx.set_tune_table(sorted((int(b), t) for b, t in tuned[name]["table"].items()),
+                           knob=knob)
 
 
 def timed_on(F, ctx, fn, reps):
@@ -453,7 +468,7 @@ def main():
         if not args.no_query:
             # the 20-bit comparison workload at its own parameter set, n0 = 636
             octx, osk, op = operator_context(F, F.PRESET_CFG636)
-            octx.set_tune_table(sorted((int(b), t) for b, t in res["autotune"]["table"].items()))
+            apply_tuning(F, octx, res["autotune"])
             res["queries"]["compare_digits_n636"] = run_compare_digits(
                 F, octx, osk, op, args.query_rows, args.query_reps)
             octx.close()

@@ -41,6 +41,9 @@ BK_FFT = 1
 PBS_FP64 = 0
 PBS_MFMA_SPLIT = 1
 
+TUNE_IKS21_TILE = 0
+TUNE_PBS1_WAVES = 1
+
 GATE_NAND, GATE_AND, GATE_OR, GATE_NOR, GATE_XNOR = range(5)
 LOGIC = 1
 ARITHMETIC = 0
@@ -413,32 +416,38 @@ class Context:
         check(lib().fhe_keygen_ksk_dev(self.h, kind, seed, _ptr(sdom), _ptr(stgt),
                                        out_dev.ptr if out_dev is not None else None))
 
-    # ---- tile autotuning (fhe_autotune / fhe_tune_get / fhe_tune_set)
-    TUNE_CANDIDATES = (8, 16, 32)
+    # ---- autotuning (fhe_autotune / fhe_tune_get / fhe_tune_set)
+    TUNE_CANDIDATES = {TUNE_IKS21_TILE: (8, 16, 32), TUNE_PBS1_WAVES: (1, 2)}
 
-    def autotune(self, batches=(64, 256, 1024, 4096, 16384)):
-        """Time the lvl2 -> lvl1 key switch's tiles (8, 16, 32 ciphertexts
-        per workgroup) at each batch size and keep the fastest; returns
-        {batch: (tile, {tile: ms})}.  Needs KSK 21."""
+    def autotune(self, batches=(64, 256, 1024, 4096, 16384), knob=None):
+        """Time every candidate of `knob` (TUNE_IKS21_TILE: ciphertexts per
+        workgroup of the lvl2 -> lvl1 key switch, needs KSK 21;
+        TUNE_PBS1_WAVES: waves per ciphertext of the lvl1 blind rotation,
+        needs BK lvl01) at each batch size and keep the fastest; returns
+        {batch: (choice, {candidate: ms})}."""
+        knob = TUNE_IKS21_TILE if knob is None else knob
+        cand = self.TUNE_CANDIDATES[knob]
         b = np.ascontiguousarray(sorted(batches), np.int32)
         choice = np.zeros(b.size, np.int32)
-        ms = np.zeros((b.size, 3), np.float32)
-        check(lib().fhe_autotune(self.h, 0, _ptr(b), int(b.size), _ptr(choice), _ptr(ms)))
-        return {int(x): (int(c), {t: float(v) for t, v in zip(self.TUNE_CANDIDATES, row)})
+        ms = np.zeros((b.size, len(cand)), np.float32)
+        check(lib().fhe_autotune(self.h, knob, _ptr(b), int(b.size), _ptr(choice), _ptr(ms)))
+        return {int(x): (int(c), {t: float(v) for t, v in zip(cand, row)})
                 for x, c, row in zip(b, choice, ms)}
 
-    def tune_table(self):
-        """[(batch, tile), ...] of the lvl2 -> lvl1 key switch ([] = untuned)."""
+    def tune_table(self, knob=None):
+        """[(batch, choice), ...] of `knob` ([] = untuned: built-in rule)."""
+        knob = TUNE_IKS21_TILE if knob is None else knob
         b = np.zeros(16, np.int32)
         t = np.zeros(16, np.int32)
         n = ctypes.c_int()
-        check(lib().fhe_tune_get(self.h, 0, _ptr(b), _ptr(t), 16, ctypes.byref(n)))
+        check(lib().fhe_tune_get(self.h, knob, _ptr(b), _ptr(t), 16, ctypes.byref(n)))
         return [(int(b[i]), int(t[i])) for i in range(n.value)]
 
-    def set_tune_table(self, table):
+    def set_tune_table(self, table, knob=None):
+        knob = TUNE_IKS21_TILE if knob is None else knob
         b = np.ascontiguousarray([x for x, _ in table], np.int32)
         t = np.ascontiguousarray([y for _, y in table], np.int32)
-        check(lib().fhe_tune_set(self.h, 0, _ptr(b), _ptr(t), int(b.size)))
+        check(lib().fhe_tune_set(self.h, knob, _ptr(b), _ptr(t), int(b.size)))
 
     def set_pbs_variant(self, variant):
         check(lib().fhe_set_pbs_variant(self.h, variant))

@@ -36,6 +36,10 @@ int ensure_mfma(fhe_ctx *ctx, hipStream_t s)
     return FHE_OK;
 }
 
+}  // namespace
+
+namespace fhe {
+
 // lvl1 blind rotation (+ sample extraction) with the context's variant
 int launch_pbs1(fhe_ctx *ctx, const uint32_t *in, const uint32_t *tv, int tv_stride,
                 uint32_t *out, bool se, int B, hipStream_t s)
@@ -49,9 +53,25 @@ int launch_pbs1(fhe_ctx *ctx, const uint32_t *in, const uint32_t *tv, int tv_str
                 "pbs lvl01 (split MFMA)");
         return FHE_OK;
     }
+    if (pbs1_waves(ctx, B) == 2) {
+        if (!ctx->bk1w2_valid) {
+            if (!ctx->bk1w2)
+                HIP_TRY(hipMalloc(&ctx->bk1w2, bk1_device_elems(n0) * sizeof(cd)),
+                        "two-wave key allocation");
+            HIP_TRY(launch_bk1_to_w2(ctx->bk1, ctx->bk1w2, n0, s), "two-wave key layout");
+            ctx->bk1w2_valid = true;
+        }
+        HIP_TRY(launch_blind_rotate1w2(in, n0, ctx->bk1w2, tv, tv_stride, out, se, B, s),
+                "pbs lvl01 (two waves per ciphertext)");
+        return FHE_OK;
+    }
     HIP_TRY(launch_blind_rotate1(in, n0, ctx->bk1, tv, tv_stride, out, se, B, s), "pbs lvl01");
     return FHE_OK;
 }
+
+}  // namespace fhe
+
+namespace {
 
 // lvl02 key given in TFHEpp's FD form: back to torus polynomials (TwistFFT,
 // FP64, as exact as the FD values allow), then into the blind rotation's
@@ -106,7 +126,7 @@ int fhe_ctx_destroy(fhe_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     void *bufs[] = {ctx->bk1,  ctx->bk2,    ctx->ksk[0], ctx->ksk[1], ctx->ksk[2],
                     ctx->trgsw1, ctx->trgsw2, ctx->tv1,  ctx->tv2,    ctx->tmp1,
-                    ctx->tmp0, ctx->tmp2,   ctx->bk1m, ctx->mfma_tab};
+                    ctx->tmp0, ctx->tmp2,   ctx->bk1m, ctx->mfma_tab, ctx->bk1w2};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (uint32_t *b : ctx->chain)
@@ -169,7 +189,7 @@ int fhe_load_bk(fhe_ctx *ctx, int level, const void *bk_host, int format)
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(tmp);
     HIP_TRY(e, "fhe_load_bk");
-    if (l1) ctx->bk1m_valid = false;
+    if (l1) ctx->bk1m_valid = ctx->bk1w2_valid = false;
     return FHE_OK;
 }
 

@@ -24,6 +24,10 @@ struct fhe_ctx {
     // on demand, bk1m_valid cleared by every lvl1 key upload) and its tables
     float *bk1m = nullptr;
     bool bk1m_valid = false;
+    // two-waves-per-ciphertext kernel: bk1 permuted into its order (rebuilt
+    // on demand, bk1w2_valid cleared by every lvl1 key upload)
+    fhe::cd *bk1w2 = nullptr;
+    bool bk1w2_valid = false;
     void *mfma_tab = nullptr;
     // grow-on-demand scratch for the bootstrap operators
     uint32_t *tv1 = nullptr;     // one lvl1 test vector
@@ -45,12 +49,13 @@ struct fhe_ctx {
     static constexpr int kDigitBufs = 4;
     uint32_t *digit[kDigitBufs] = {};
     size_t digit_cap[kDigitBufs] = {};
-    // tuned tile of the lvl2 -> lvl1 key switch per batch size (ascending
-    // batch sizes; a batch uses the entry of the largest size <= it)
+    // tuned choices per knob (FHE_TUNE_*) and batch size (ascending batch
+    // sizes; a batch uses the entry of the largest size <= it)
+    static constexpr int kTuneKnobs = 2;
     static constexpr int kTuneMax = 16;
-    int tune21_batch[kTuneMax] = {};
-    int tune21_tile[kTuneMax] = {};
-    int tune21_n = 0;
+    int tune_batch[kTuneKnobs][kTuneMax] = {};
+    int tune_val[kTuneKnobs][kTuneMax] = {};
+    int tune_n[kTuneKnobs] = {};
 };
 
 namespace fhe {
@@ -73,6 +78,9 @@ inline hipStream_t pick(fhe_ctx *ctx, void *stream)
 {
     return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
 }
+
+// untuned: batches below this many ciphertexts use two waves per ciphertext
+constexpr int kPbs1TwoWaveBelow = 1024;
 
 // set on this thread between fhe_graph_begin and fhe_graph_end: scratch
 // must not be (re)allocated while the context stream is being captured
@@ -108,6 +116,16 @@ inline int alloc_ksk(fhe_ctx *ctx, int kind)
     return FHE_OK;
 }
 
+// tuned value of `knob` for a batch of B, or `dflt` if the knob is untuned
+inline int tuned(const fhe_ctx *ctx, int knob, int B, int dflt)
+{
+    const int n = ctx->tune_n[knob];
+    if (n == 0) return dflt;
+    int v = ctx->tune_val[knob][0];
+    for (int i = 1; i < n && ctx->tune_batch[knob][i] <= B; i++) v = ctx->tune_val[knob][i];
+    return v;
+}
+
 // workgroup tile of a key switch of B ciphertexts (only kind 21 has a
 // choice).  Untuned: 8 below 512 ciphertexts, 16 below 2048, else 32 (one
 // MI355X: 32 ciphertexts 1.9 / 2.7 / 9.2 ms with tiles 8 / 16 / 32, 2048
@@ -115,11 +133,19 @@ inline int alloc_ksk(fhe_ctx *ctx, int kind)
 inline int iks_tile(const fhe_ctx *ctx, int kind, int B)
 {
     if (kind != 21) return 0;
-    if (ctx->tune21_n == 0) return B < 512 ? 8 : B < 2048 ? 16 : 32;
-    int t = ctx->tune21_tile[0];
-    for (int i = 1; i < ctx->tune21_n && ctx->tune21_batch[i] <= B; i++) t = ctx->tune21_tile[i];
-    return t;
+    return tuned(ctx, FHE_TUNE_IKS21_TILE, B, B < 512 ? 8 : B < 2048 ? 16 : 32);
 }
+
+// waves per ciphertext of the FP64 lvl1 blind rotation for a batch of B
+inline int pbs1_waves(const fhe_ctx *ctx, int B)
+{
+    return tuned(ctx, FHE_TUNE_PBS1_WAVES, B, B < kPbs1TwoWaveBelow ? 2 : 1);
+}
+
+// lvl1 blind rotation (+ sample extraction) of B ciphertexts with the
+// context's variant and tuned shape (capi.hip)
+int launch_pbs1(fhe_ctx *ctx, const uint32_t *in, const uint32_t *tv, int tv_stride,
+                uint32_t *out, bool se, int B, hipStream_t s);
 
 inline int ks_target_n(const fhe_ctx *ctx, int kind) { return kind == 21 ? Lvl1::N : int(ctx->p.n0); }
 inline int ks_domain_words(int kind) { return kind == 10 ? Lvl1::N + 1 : Lvl2::N + 1; }

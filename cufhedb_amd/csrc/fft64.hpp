@@ -269,6 +269,82 @@ __device__ __forceinline__ void fft512_inv(cd v[8], cd *lds, int lane, const Twi
     }
 }
 
+constexpr int kLaneTile = 320;  // double2 per wave tile of lanes64_fwd / lanes64_inv
+
+// v[q] *= base^q (CONJ: conj(base)^q), q = 1..3
+template <bool CONJ>
+__device__ __forceinline__ void twiddle4(cd v[4], cd base)
+{
+    const cd b2 = cmul(base, base);
+    const cd b3 = cmul(b2, base);
+    v[1] = CONJ ? cmulc(v[1], base) : cmul(v[1], base);
+    v[2] = CONJ ? cmulc(v[2], b2) : cmul(v[2], b2);
+    v[3] = CONJ ? cmulc(v[3], b3) : cmul(v[3], b3);
+}
+
+// 64-point DFT over the lanes of a wave for each of the four register
+// sequences, as three radix-4 passes through a wave-private LDS tile of
+// kLaneTile double2 (row strides 64, 20 and 5, conflict-free for 16-lane
+// b128 accesses).  t64 = W64^(lane & 15), t16 = W16^(lane & 3).  Input:
+// register r, lane L holds x_r[L]; output: register q, lane l holds
+// X_r[m] for r = l >> 4, m = ((l >> 2) & 3) + 4 (l & 3) + 16 q.
+__device__ __forceinline__ void lanes64_fwd(cd v[4], cd *tile, int l, cd t64, cd t16)
+{
+    // T1: (lane L, reg m1) -> lane (L0 = l & 15, m1 = l >> 4), reg L1
+#pragma unroll
+    for (int m1 = 0; m1 < 4; m1++) tile[m1 * 64 + l] = v[m1];
+    wave_lds_sync();
+#pragma unroll
+    for (int L1 = 0; L1 < 4; L1++) v[L1] = tile[(l >> 4) * 64 + (l & 15) + 16 * L1];
+    wave_lds_sync();
+    dft4<1>(v[0], v[1], v[2], v[3]);
+    twiddle4<false>(v, t64);
+    // T2: (lane (L0, m1), reg q1) -> lane (L00 = l & 3, rest = l >> 2 = 4 m1 + q1), reg L01
+#pragma unroll
+    for (int q1 = 0; q1 < 4; q1++) tile[((l >> 4) * 4 + q1) * 20 + (l & 15)] = v[q1];
+    wave_lds_sync();
+#pragma unroll
+    for (int L01 = 0; L01 < 4; L01++) v[L01] = tile[(l >> 2) * 20 + (l & 3) + 4 * L01];
+    wave_lds_sync();
+    dft4<1>(v[0], v[1], v[2], v[3]);
+    twiddle4<false>(v, t16);
+    // T3: (lane (L00, rest), reg q21) -> lane 4 rest + q21, reg L00
+#pragma unroll
+    for (int q21 = 0; q21 < 4; q21++) tile[((l >> 2) * 4 + q21) * 5 + (l & 3)] = v[q21];
+    wave_lds_sync();
+#pragma unroll
+    for (int L00 = 0; L00 < 4; L00++) v[L00] = tile[l * 5 + L00];
+    wave_lds_sync();
+    dft4<1>(v[0], v[1], v[2], v[3]);
+}
+
+__device__ __forceinline__ void lanes64_inv(cd v[4], cd *tile, int l, cd t64, cd t16)
+{
+    dft4<-1>(v[0], v[1], v[2], v[3]);
+#pragma unroll
+    for (int L00 = 0; L00 < 4; L00++) tile[l * 5 + L00] = v[L00];
+    wave_lds_sync();
+#pragma unroll
+    for (int q21 = 0; q21 < 4; q21++) v[q21] = tile[((l >> 2) * 4 + q21) * 5 + (l & 3)];
+    wave_lds_sync();
+    twiddle4<true>(v, t16);
+    dft4<-1>(v[0], v[1], v[2], v[3]);
+#pragma unroll
+    for (int L01 = 0; L01 < 4; L01++) tile[(l >> 2) * 20 + (l & 3) + 4 * L01] = v[L01];
+    wave_lds_sync();
+#pragma unroll
+    for (int q1 = 0; q1 < 4; q1++) v[q1] = tile[((l >> 4) * 4 + q1) * 20 + (l & 15)];
+    wave_lds_sync();
+    twiddle4<true>(v, t64);
+    dft4<-1>(v[0], v[1], v[2], v[3]);
+#pragma unroll
+    for (int L1 = 0; L1 < 4; L1++) tile[(l >> 4) * 64 + (l & 15) + 16 * L1] = v[L1];
+    wave_lds_sync();
+#pragma unroll
+    for (int m1 = 0; m1 < 4; m1++) v[m1] = tile[m1 * 64 + l];
+    wave_lds_sync();
+}
+
 // frequency index held by (lane s, register q) in the permuted layout
 __host__ __device__ constexpr int fd_index1024(int s, int q) { return (s >> 3) + 8 * (s & 7) + 64 * q; }
 

@@ -26,6 +26,13 @@ hipError_t launch_fft1(const double *fd, uint32_t *poly, int B, hipStream_t s);
 hipError_t launch_sample_extract1(const uint32_t *trlwe, uint32_t *tlwe, int index, int B,
                                   hipStream_t s);
 
+// ---- lvl1, two waves per ciphertext (small batches), pbs_lvl1w2.hip -----
+// key: the one-wave device key with each (i, row, col) slice permuted
+hipError_t launch_bk1_to_w2(const cd *bk1, cd *bk_w2, int n0, hipStream_t s);
+hipError_t launch_blind_rotate1w2(const uint32_t *in, int n0, const cd *bk, const uint32_t *tv,
+                                  int tv_stride, uint32_t *out, bool sample_extract, int B,
+                                  hipStream_t s);
+
 // ---- lvl1 split-MFMA variant, pbs_lvl1_mfma.hip -------------------------
 size_t mfma_tables_bytes();
 size_t bk1_mfma_floats(int n0);

@@ -33,7 +33,7 @@ namespace {
 
 constexpr int N2q = Lvl2::N;
 constexpr uint64_t kDecOff2q = decomp_total_offset<uint64_t, 64, Lvl2::l, Lvl2::Bgbit>();
-constexpr int kTileQ = 320;          // double2 per wave tile (strides 64 / 20 / 5)
+constexpr int kTileQ = kLaneTile;    // double2 per wave tile
 constexpr double kKeyScale2 = 1.0 / 1024;
 
 // per-lane twiddle bases
@@ -61,76 +61,6 @@ __device__ __forceinline__ cd cq(int p)
 {
     constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173;
     return p == 1 ? cd{c1, s1} : p == 2 ? cd{kR2, kR2} : cd{s1, c1};
-}
-
-// v[q] *= base^q (CONJ: conj(base)^q), q = 1..3
-template <bool CONJ>
-__device__ __forceinline__ void twiddle4(cd v[4], cd base)
-{
-    const cd b2 = cmul(base, base);
-    const cd b3 = cmul(b2, base);
-    v[1] = CONJ ? cmulc(v[1], base) : cmul(v[1], base);
-    v[2] = CONJ ? cmulc(v[2], b2) : cmul(v[2], b2);
-    v[3] = CONJ ? cmulc(v[3], b3) : cmul(v[3], b3);
-}
-
-// 64-point DFT over the lanes of a wave for the four register sequences,
-// as three radix-4 passes; tile is this wave's (kTileQ double2)
-__device__ __forceinline__ void lanes64_fwd(cd v[4], cd *tile, int l, const TwQ &tw)
-{
-    // T1: (lane L, reg m1) -> lane (L0 = l & 15, m1 = l >> 4), reg L1
-#pragma unroll
-    for (int m1 = 0; m1 < 4; m1++) tile[m1 * 64 + l] = v[m1];
-    wave_lds_sync();
-#pragma unroll
-    for (int L1 = 0; L1 < 4; L1++) v[L1] = tile[(l >> 4) * 64 + (l & 15) + 16 * L1];
-    wave_lds_sync();
-    dft4<1>(v[0], v[1], v[2], v[3]);
-    twiddle4<false>(v, tw.t64);
-    // T2: (lane (L0, m1), reg q1) -> lane (L00 = l & 3, rest = l >> 2 = 4 m1 + q1), reg L01
-#pragma unroll
-    for (int q1 = 0; q1 < 4; q1++) tile[((l >> 4) * 4 + q1) * 20 + (l & 15)] = v[q1];
-    wave_lds_sync();
-#pragma unroll
-    for (int L01 = 0; L01 < 4; L01++) v[L01] = tile[(l >> 2) * 20 + (l & 3) + 4 * L01];
-    wave_lds_sync();
-    dft4<1>(v[0], v[1], v[2], v[3]);
-    twiddle4<false>(v, tw.t16);
-    // T3: (lane (L00, rest), reg q21) -> lane 4 rest + q21, reg L00
-#pragma unroll
-    for (int q21 = 0; q21 < 4; q21++) tile[((l >> 2) * 4 + q21) * 5 + (l & 3)] = v[q21];
-    wave_lds_sync();
-#pragma unroll
-    for (int L00 = 0; L00 < 4; L00++) v[L00] = tile[l * 5 + L00];
-    wave_lds_sync();
-    dft4<1>(v[0], v[1], v[2], v[3]);
-}
-
-__device__ __forceinline__ void lanes64_inv(cd v[4], cd *tile, int l, const TwQ &tw)
-{
-    dft4<-1>(v[0], v[1], v[2], v[3]);
-#pragma unroll
-    for (int L00 = 0; L00 < 4; L00++) tile[l * 5 + L00] = v[L00];
-    wave_lds_sync();
-#pragma unroll
-    for (int q21 = 0; q21 < 4; q21++) v[q21] = tile[((l >> 2) * 4 + q21) * 5 + (l & 3)];
-    wave_lds_sync();
-    twiddle4<true>(v, tw.t16);
-    dft4<-1>(v[0], v[1], v[2], v[3]);
-#pragma unroll
-    for (int L01 = 0; L01 < 4; L01++) tile[(l >> 2) * 20 + (l & 3) + 4 * L01] = v[L01];
-    wave_lds_sync();
-#pragma unroll
-    for (int q1 = 0; q1 < 4; q1++) v[q1] = tile[((l >> 4) * 4 + q1) * 20 + (l & 15)];
-    wave_lds_sync();
-    twiddle4<true>(v, tw.t64);
-    dft4<-1>(v[0], v[1], v[2], v[3]);
-#pragma unroll
-    for (int L1 = 0; L1 < 4; L1++) tile[(l >> 4) * 64 + (l & 15) + 16 * L1] = v[L1];
-    wave_lds_sync();
-#pragma unroll
-    for (int m1 = 0; m1 < 4; m1++) v[m1] = tile[m1 * 64 + l];
-    wave_lds_sync();
 }
 
 // Forward: v[p] = z[lane + 64 wave + 256 p] in, Z[4 m' + wave] (layout above)
@@ -162,12 +92,12 @@ __device__ __forceinline__ void fft1024q_fwd(cd v[4], cd *xch, int lane, int wav
     // step C: radix-4 over the registers (stride 64), twiddle W256^{L m1}
     dft4<1>(v[0], v[1], v[2], v[3]);
     twiddle4<false>(v, tw.t256);
-    lanes64_fwd(v, xch + wave * kTileQ, lane, tw);
+    lanes64_fwd(v, xch + wave * kTileQ, lane, tw.t64, tw.t16);
 }
 
 __device__ __forceinline__ void fft1024q_inv(cd v[4], cd *xch, int lane, int wave, const TwQ &tw)
 {
-    lanes64_inv(v, xch + wave * kTileQ, lane, tw);
+    lanes64_inv(v, xch + wave * kTileQ, lane, tw.t64, tw.t16);
     twiddle4<true>(v, tw.t256);
     dft4<-1>(v[0], v[1], v[2], v[3]);
     // all waves finished their tiles before the exchange overwrites them

@@ -306,7 +306,13 @@ int fhe_digit_testvectors(int op, int digits, int digit_bits, uint64_t threshold
  * picks, ms (n x 3 in candidate order 8, 16, 32, may be NULL) the mean
  * times.  All candidates compute identical results.  fhe_tune_get /
  * fhe_tune_set read and restore the table (e.g. from a per-device cache). */
-enum fhe_tune_knob { FHE_TUNE_IKS21_TILE = 0 };
+enum fhe_tune_knob {
+    FHE_TUNE_IKS21_TILE = 0, /* ciphertexts per workgroup: 8, 16, 32          */
+    FHE_TUNE_PBS1_WAVES = 1  /* waves per ciphertext of the FP64 lvl1 blind
+                                rotation: 1, or 2 (small batches: each
+                                ciphertext's CMUX on two SIMDs); untuned: 2
+                                below 1024 ciphertexts */
+};
 int fhe_autotune(fhe_ctx *ctx, int knob, const int *batches, int n, int *choice, float *ms);
 int fhe_tune_set(fhe_ctx *ctx, int knob, const int *batches, const int *tiles, int n);
 int fhe_tune_get(fhe_ctx *ctx, int knob, int *batches, int *tiles, int cap, int *n);
