@@ -131,15 +131,19 @@ TUNE = {
                    (32, 128, 512, 1024, 2048, 4096, 16384)),
     "pbs1_waves": ("waves per ciphertext of the lvl1 blind rotation",
                    (64, 256, 512, 1024, 2048)),
+    "iks_split": ("workgroups per ciphertext of the lvl1 -> lvl0 key switch",
+                  (16, 64, 256, 1024, 4096, 16384)),
 }
 
 
 def autotune(F, ctx):
     """Shapes measured on this GPU per batch size and kept in the context for
-    the query pipelines: the lvl2 -> lvl1 key switch's tile and the lvl1
-    blind rotation's waves per ciphertext."""
+    the query pipelines: the lvl2 -> lvl1 key switch's tile, the lvl1 blind
+    rotation's waves per ciphertext and the lvl1/lvl2 -> lvl0 key switches'
+    workgroups per ciphertext."""
     out = {}
-    for name, knob in (("iks21_tile", F.TUNE_IKS21_TILE), ("pbs1_waves", F.TUNE_PBS1_WAVES)):
+    for name, knob in (("iks21_tile", F.TUNE_IKS21_TILE), ("pbs1_waves", F.TUNE_PBS1_WAVES),
+                       ("iks_split", F.TUNE_IKS_SPLIT)):
         what, batches = TUNE[name]
         res = ctx.autotune(batches, knob=knob)
         print(f"bench: autotuned {what}: {ctx.tune_table(knob=knob)}", file=sys.stderr,
@@ -150,7 +154,8 @@ def autotune(F, ctx):
 
 
 def apply_tuning(F, ctx, tuned):
-    for name, knob in (("iks21_tile", F.TUNE_IKS21_TILE), ("pbs1_waves", F.TUNE_PBS1_WAVES)):
+    for name, knob in (("iks21_tile", F.TUNE_IKS21_TILE), ("pbs1_waves", F.TUNE_PBS1_WAVES),
+                       ("iks_split", F.TUNE_IKS_SPLIT)):
         ctx.set_tune_table(sorted((int(b), t) for b, t in tuned[name]["table"].items()),
                            knob=knob)
 

@@ -25,14 +25,15 @@ namespace {
 
 struct Knob {
     int ncand;
-    int cand[3];
+    int cand[4];
 };
 
 bool knob_of(int knob, Knob &k)
 {
     switch (knob) {
-    case FHE_TUNE_IKS21_TILE: k = {3, {8, 16, 32}}; return true;
-    case FHE_TUNE_PBS1_WAVES: k = {2, {1, 2, 0}}; return true;
+    case FHE_TUNE_IKS21_TILE: k = {3, {8, 16, 32, 0}}; return true;
+    case FHE_TUNE_PBS1_WAVES: k = {2, {1, 2, 0, 0}}; return true;
+    case FHE_TUNE_IKS_SPLIT: k = {4, {1, 2, 4, 8}}; return true;
     default: return false;
     }
 }
@@ -109,23 +110,33 @@ int fhe_autotune(fhe_ctx *ctx, int knob, const int *batches, int n, int *choice,
         FHE_CHECK_ARG(batches[i] >= 1 && batches[i] <= (1 << 20) &&
                           (i == 0 || batches[i] > batches[i - 1]),
                       "fhe_autotune: batch sizes must be ascending, in [1, 2^20]");
-    const bool iks = knob == FHE_TUNE_IKS21_TILE;
-    if (iks ? !ctx->ksk[ksk_slot(21)] : !ctx->bk1) {
-        set_error("fhe_autotune: %s not loaded", iks ? "KSK 21" : "BK lvl01");
+    // the kernel each knob shapes, and the key it needs: KSK 21; KSK 10 (or
+    // else 20) for the digit-position split; BK lvl01
+    const int iks_kind = knob == FHE_TUNE_IKS21_TILE ? 21
+                         : knob == FHE_TUNE_IKS_SPLIT ? (ctx->ksk[ksk_slot(10)] ? 10 : 20)
+                                                      : 0;
+    const bool iks = iks_kind != 0;
+    if (iks ? !ctx->ksk[ksk_slot(iks_kind)] : !ctx->bk1) {
+        set_error("fhe_autotune: %s not loaded",
+                  knob == FHE_TUNE_IKS21_TILE ? "KSK 21"
+                  : knob == FHE_TUNE_IKS_SPLIT ? "KSK 10 or 20" : "BK lvl01");
         return FHE_ERR_NOT_LOADED;
     }
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     hipStream_t s = ctx->stream;
     const int bmax = batches[n - 1];
     const int n0 = int(ctx->p.n0);
-    // inputs: lvl2 TLWEs (key switch) or lvl0 TLWEs plus one test vector
-    const size_t in_words = iks ? size_t(bmax) * (Lvl2::N + 1)
-                                : (size_t(bmax) * (n0 + 1) + Lvl1::N + 1) / 2;
+    // inputs (64-bit words): lvl2 TLWEs, lvl1 TLWEs, or lvl0 TLWEs plus one
+    // test vector
+    const size_t in_words = iks_kind == 10 ? (size_t(bmax) * (Lvl1::N + 1) + 1) / 2
+                            : iks          ? size_t(bmax) * (Lvl2::N + 1)
+                                           : (size_t(bmax) * (n0 + 1) + Lvl1::N + 1) / 2;
     uint64_t *in = nullptr;
     uint32_t *out = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipMalloc(&in, in_words * sizeof(uint64_t));
-    if (e == hipSuccess) e = hipMalloc(&out, size_t(bmax) * (Lvl1::N + 1) * sizeof(uint32_t));
+    const int tn = iks_kind == 21 ? Lvl1::N : iks ? n0 : Lvl1::N;  // output TLWE width - 1
+    if (e == hipSuccess) e = hipMalloc(&out, size_t(bmax) * (tn + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) {
@@ -144,8 +155,9 @@ int fhe_autotune(fhe_ctx *ctx, int knob, const int *batches, int n, int *choice,
         float best = 0.f;
         for (int c = 0; c < k.ncand && e == hipSuccess; c++) {
             auto run = [&]() -> hipError_t {
-                if (iks) return launch_iks(21, in, ctx->ksk[ksk_slot(21)], out, Lvl1::N, B, s,
-                                           k.cand[c]);
+                if (iks)
+                    return launch_iks(iks_kind, in, ctx->ksk[ksk_slot(iks_kind)], out, tn, B, s,
+                                      k.cand[c]);
                 // the lvl1 blind rotation with this candidate forced
                 ctx->tune_batch[knob][0] = 1;
                 ctx->tune_val[knob][0] = k.cand[c];

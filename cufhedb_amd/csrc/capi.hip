@@ -402,7 +402,8 @@ int run_bootstrap(fhe_ctx *ctx, int in_level, const void *in, int out_level, voi
     int rc = grow(ctx->tmp0, ctx->tmp0_cap, size_t(B) * (ctx->p.n0 + 1));
     if (rc) return rc;
     const int n0 = int(ctx->p.n0);
-    HIP_TRY(launch_iks(kind, in, ctx->ksk[slot], ctx->tmp0, n0, B, s), "IKS");
+    HIP_TRY(launch_iks(kind, in, ctx->ksk[slot], ctx->tmp0, n0, B, s, iks_tile(ctx, kind, B)),
+            "IKS");
     if (out_level == 1) {
         HIP_TRY(tv.ide ? launch_tv_ide<uint32_t>(ctx->tv1, uint32_t(tv.value), Lvl1::N, s)
                        : launch_tv_const<uint32_t>(ctx->tv1, uint32_t(tv.value), Lvl1::N, s),

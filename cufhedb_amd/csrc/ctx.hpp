@@ -51,7 +51,7 @@ struct fhe_ctx {
     size_t digit_cap[kDigitBufs] = {};
     // tuned choices per knob (FHE_TUNE_*) and batch size (ascending batch
     // sizes; a batch uses the entry of the largest size <= it)
-    static constexpr int kTuneKnobs = 2;
+    static constexpr int kTuneKnobs = 3;
     static constexpr int kTuneMax = 16;
     int tune_batch[kTuneKnobs][kTuneMax] = {};
     int tune_val[kTuneKnobs][kTuneMax] = {};
@@ -126,14 +126,17 @@ inline int tuned(const fhe_ctx *ctx, int knob, int B, int dflt)
     return v;
 }
 
-// workgroup tile of a key switch of B ciphertexts (only kind 21 has a
-// choice).  Untuned: 8 below 512 ciphertexts, 16 below 2048, else 32 (one
-// MI355X: 32 ciphertexts 1.9 / 2.7 / 9.2 ms with tiles 8 / 16 / 32, 2048
-// ciphertexts 19.1 / 17.3 / 13.6 ms)
+// workgroup shape of a key switch of B ciphertexts.  Kind 21: ciphertexts
+// per workgroup; untuned 8 below 512 ciphertexts, 16 below 2048, else 32
+// (one MI355X: 32 ciphertexts 1.9 / 2.7 / 9.2 ms with tiles 8 / 16 / 32,
+// 2048 ciphertexts 19.1 / 17.3 / 13.6 ms).  Kinds 10 and 20: workgroups per
+// ciphertext over the digit positions; untuned 8 (one MI355X, n0 = 672, split
+// 1 / 8: kind 10 16 ciphertexts 0.13 / 0.02 ms, 16384 ciphertexts 15.1 /
+// 13.5 ms; kind 20 16384 ciphertexts 31.0 / 28.6 ms).
 inline int iks_tile(const fhe_ctx *ctx, int kind, int B)
 {
-    if (kind != 21) return 0;
-    return tuned(ctx, FHE_TUNE_IKS21_TILE, B, B < 512 ? 8 : B < 2048 ? 16 : 32);
+    if (kind == 21) return tuned(ctx, FHE_TUNE_IKS21_TILE, B, B < 512 ? 8 : B < 2048 ? 16 : 32);
+    return tuned(ctx, FHE_TUNE_IKS_SPLIT, B, 8);
 }
 
 // waves per ciphertext of the FP64 lvl1 blind rotation for a batch of B

@@ -37,16 +37,19 @@ __global__ __launch_bounds__(256) void k_iks_rows(const TD *__restrict__ in,
 {
     constexpr int W = int(sizeof(TD) * 8);
     constexpr int NDIG = DN * T;
-    static_assert(NDIG % 8 == 0, "digit positions are taken in pairs by four waves");
+    static_assert(NDIG % 64 == 0, "digit positions are split in multiples of 8 pairs");
     constexpr uint32_t mask = (1u << BB) - 1;
     constexpr size_t nd = (size_t(1) << BB) - 1;
     __shared__ uint16_t dig[NDIG];
     __shared__ uint32_t part[4][NCH * 64];
     const int b = blockIdx.x;
     if (b >= B) return;
+    // split gridDim.y ways over the digit positions: this workgroup takes
+    // [e0, e0 + per) and adds its sum into the zero-filled output
+    const int nsplit = gridDim.y, per = NDIG / nsplit, e0 = blockIdx.y * per;
     const TD *c = in + size_t(b) * (DN + 1);
     const TD prec = TD(1) << (W - (1 + BB * T));
-    for (int e = threadIdx.x; e < NDIG; e += blockDim.x) {
+    for (int e = e0 + threadIdx.x; e < e0 + per; e += blockDim.x) {
         const int i = e / T, j = e % T;
         dig[e] = uint16_t(((c[i] + prec) >> (W - (j + 1) * BB)) & mask);
     }
@@ -57,7 +60,7 @@ __global__ __launch_bounds__(256) void k_iks_rows(const TD *__restrict__ in,
     uint32_t acc[NCH];
 #pragma unroll
     for (int j = 0; j < NCH; j++) acc[j] = 0;
-    for (int e = wave; e < NDIG; e += 8) {
+    for (int e = e0 + wave; e < e0 + per; e += 8) {
         const uint32_t d0 = __builtin_amdgcn_readfirstlane(uint32_t(dig[e]));
         const uint32_t d1 = __builtin_amdgcn_readfirstlane(uint32_t(dig[e + 4]));
         const uint32_t keep0 = d0 ? 0xffffffffu : 0u, keep1 = d1 ? 0xffffffffu : 0u;
@@ -80,19 +83,41 @@ __global__ __launch_bounds__(256) void k_iks_rows(const TD *__restrict__ in,
     else
         body = uint32_t((c[DN] + (TD(1) << (W - 33))) >> (W - 32));
     uint32_t *o = out + size_t(b) * row;
-    for (int k = threadIdx.x; k < int(row); k += blockDim.x)
-        o[k] = (k == tn ? body : 0u) + part[0][k] + part[1][k] + part[2][k] + part[3][k];
+    if (nsplit == 1) {
+        for (int k = threadIdx.x; k < int(row); k += blockDim.x)
+            o[k] = (k == tn ? body : 0u) + part[0][k] + part[1][k] + part[2][k] + part[3][k];
+    } else {
+        // integer sums mod 2^32: the order of the workgroups' additions does
+        // not change the result
+        for (int k = threadIdx.x; k < int(row); k += blockDim.x)
+            atomicAdd(o + k, (k == tn && blockIdx.y == 0 ? body : 0u) + part[0][k] + part[1][k] +
+                                 part[2][k] + part[3][k]);
+    }
 }
 
-// launch k_iks_rows with NCH = ceil((tn + 1) / 64), tn <= kMaxN0
+__global__ void k_fill_zero32(uint32_t *p, size_t n)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// launch k_iks_rows with NCH = ceil((tn + 1) / 64), tn <= kMaxN0, split
+// ways over the digit positions (1, 2, 4 or 8; a split output is zeroed by a
+// kernel first, so the launch stays capturable)
 template <class TD, int DN, int T, int BB, int NCH = 1>
-void launch_iks_rows(const TD *in, const uint32_t *ksk, uint32_t *out, int tn, int B, hipStream_t s)
+void launch_iks_rows(const TD *in, const uint32_t *ksk, uint32_t *out, int tn, int B, int split,
+                     hipStream_t s)
 {
     if constexpr (NCH < (kMaxN0 + 1 + 63) / 64) {
-        if ((tn + 1 + 63) / 64 > NCH) return launch_iks_rows<TD, DN, T, BB, NCH + 1>(in, ksk, out, tn, B, s);
+        if ((tn + 1 + 63) / 64 > NCH)
+            return launch_iks_rows<TD, DN, T, BB, NCH + 1>(in, ksk, out, tn, B, split, s);
     }
-    hipLaunchKernelGGL((k_iks_rows<TD, DN, T, BB, NCH>), dim3(B), dim3(256), 0, s, in, ksk, out,
-                       tn, B);
+    if (split > 1) {
+        const size_t n = size_t(B) * (tn + 1);
+        hipLaunchKernelGGL(k_fill_zero32, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, out, n);
+    }
+    hipLaunchKernelGGL((k_iks_rows<TD, DN, T, BB, NCH>), dim3(B, split), dim3(256), 0, s, in, ksk,
+                       out, tn, B);
 }
 
 // Tiled form for small bases (lvl21: basebit 3, 7 key rows per digit
@@ -350,13 +375,17 @@ hipError_t launch_iks(int kind, const void *in, const uint32_t *ksk, uint32_t *o
     if (B <= 0) return hipSuccess;
     switch (kind) {
     case 10:
-        launch_iks_rows<uint32_t, Lvl1::N, KS10::t, KS10::basebit>(
-            static_cast<const uint32_t *>(in), ksk, out, tn, B, s);
+    case 20: {
+        // `tile` = workgroups per ciphertext over the digit positions
+        const int split = tile == 2 || tile == 4 || tile == 8 ? tile : 1;
+        if (kind == 10)
+            launch_iks_rows<uint32_t, Lvl1::N, KS10::t, KS10::basebit>(
+                static_cast<const uint32_t *>(in), ksk, out, tn, B, split, s);
+        else
+            launch_iks_rows<uint64_t, Lvl2::N, KS20::t, KS20::basebit>(
+                static_cast<const uint64_t *>(in), ksk, out, tn, B, split, s);
         break;
-    case 20:
-        launch_iks_rows<uint64_t, Lvl2::N, KS20::t, KS20::basebit>(
-            static_cast<const uint64_t *>(in), ksk, out, tn, B, s);
-        break;
+    }
     case 21: {
         // `tile` ciphertexts per workgroup (8, 16 or 32; 0 = 32), all 10 digit
         // rows of a mask word in flight.  Larger tiles read each key row for

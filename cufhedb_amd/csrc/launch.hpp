@@ -64,7 +64,9 @@ hipError_t launch_sample_extract2(const uint64_t *trlwe, uint64_t *tlwe, int ind
 
 // ---- identity key switching and small element-wise ops, iks.hip ---------
 // kind 10: uint32 domain (dn = 1024), kind 20/21: uint64 domain (dn = 2048)
-// tile: ciphertexts per workgroup of the kind-21 kernel (8, 16, 32; 0 = 32)
+// tile: kind 21, ciphertexts per workgroup (8, 16, 32; 0 = 32); kinds 10
+// and 20, workgroups per ciphertext over the digit positions (1, 2, 4, 8;
+// 0 = 1)
 hipError_t launch_iks(int kind, const void *in, const uint32_t *ksk, uint32_t *out, int tn, int B,
                       hipStream_t s, int tile = 0);
 // out = sa*a + sb*b (+ offset on the body word), lvl1 TLWEs of 1025 words

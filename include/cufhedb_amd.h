@@ -308,10 +308,14 @@ int fhe_digit_testvectors(int op, int digits, int digit_bits, uint64_t threshold
  * fhe_tune_set read and restore the table (e.g. from a per-device cache). */
 enum fhe_tune_knob {
     FHE_TUNE_IKS21_TILE = 0, /* ciphertexts per workgroup: 8, 16, 32          */
-    FHE_TUNE_PBS1_WAVES = 1  /* waves per ciphertext of the FP64 lvl1 blind
+    FHE_TUNE_PBS1_WAVES = 1, /* waves per ciphertext of the FP64 lvl1 blind
                                 rotation: 1, or 2 (small batches: each
                                 ciphertext's CMUX on two SIMDs); untuned: 2
                                 below 1024 ciphertexts */
+    FHE_TUNE_IKS_SPLIT = 2   /* workgroups per ciphertext of the lvl1/lvl2 ->
+                                lvl0 key switches (1, 2, 4, 8: the digit
+                                positions split, sums added with integer
+                                atomics); untuned: 8 */
 };
 int fhe_autotune(fhe_ctx *ctx, int knob, const int *batches, int n, int *choice, float *ms);
 int fhe_tune_set(fhe_ctx *ctx, int knob, const int *batches, const int *tiles, int n);
