@@ -126,7 +126,7 @@ int fhe_ctx_destroy(fhe_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     void *bufs[] = {ctx->bk1,  ctx->bk2,    ctx->ksk[0], ctx->ksk[1], ctx->ksk[2],
                     ctx->trgsw1, ctx->trgsw2, ctx->tv1,  ctx->tv2,    ctx->tmp1,
-                    ctx->tmp0, ctx->tmp2,   ctx->bk1m, ctx->mfma_tab, ctx->bk1w2};
+                    ctx->tmp0, ctx->tmp2,   ctx->bk1m, ctx->mfma_tab, ctx->bk1w2, ctx->rot2};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (uint32_t *b : ctx->chain)
@@ -186,6 +186,12 @@ int fhe_load_bk(fhe_ctx *ctx, int level, const void *bk_host, int format)
                     ? launch_bk2q_from_int(static_cast<uint64_t *>(tmp), dst, n0, ctx->stream)
                     : bk2_fft_to_q(static_cast<double *>(tmp), dst, n0, ctx->stream);
     }
+    // the lvl2 blind rotation applies X^a - 1 in the frequency domain with
+    // this table of w2^e
+    if (e == hipSuccess && !l1 && !ctx->rot2) {
+        e = hipMalloc(&ctx->rot2, rot2q_elems() * sizeof(cd));
+        if (e == hipSuccess) e = launch_rot2q_table(ctx->rot2, ctx->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(tmp);
     HIP_TRY(e, "fhe_load_bk");
@@ -241,7 +247,7 @@ int fhe_pbs_lvl02_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint64_t *tv_d
         set_error("fhe_pbs_lvl02_dev: lvl02 bootstrapping key not loaded");
         return FHE_ERR_NOT_LOADED;
     }
-    HIP_TRY(launch_blind_rotate2q(in_dev, int(ctx->p.n0), ctx->bk2, tv_dev, tv_stride, out_dev, true,
+    HIP_TRY(launch_blind_rotate2q(in_dev, int(ctx->p.n0), ctx->bk2, ctx->rot2, tv_dev, tv_stride, out_dev, true,
                                  B, pick(ctx, stream)),
             "pbs lvl02");
     return FHE_OK;
@@ -257,7 +263,7 @@ int fhe_blind_rotate_lvl02_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint6
         set_error("fhe_blind_rotate_lvl02_dev: lvl02 bootstrapping key not loaded");
         return FHE_ERR_NOT_LOADED;
     }
-    HIP_TRY(launch_blind_rotate2q(in_dev, int(ctx->p.n0), ctx->bk2, tv_dev, tv_stride, acc_dev,
+    HIP_TRY(launch_blind_rotate2q(in_dev, int(ctx->p.n0), ctx->bk2, ctx->rot2, tv_dev, tv_stride, acc_dev,
                                  false, B, pick(ctx, stream)),
             "blind rotate lvl02");
     return FHE_OK;
@@ -418,7 +424,7 @@ int run_bootstrap(fhe_ctx *ctx, int in_level, const void *in, int out_level, voi
         HIP_TRY(tv.ide ? launch_tv_ide<uint64_t>(ctx->tv2, uint32_t(tv.value), Lvl2::N, s)
                        : launch_tv_const<uint64_t>(ctx->tv2, tv.value, Lvl2::N, s),
                 "test vector");
-        HIP_TRY(launch_blind_rotate2q(ctx->tmp0, n0, ctx->bk2, ctx->tv2, 0,
+        HIP_TRY(launch_blind_rotate2q(ctx->tmp0, n0, ctx->bk2, ctx->rot2, ctx->tv2, 0,
                                      static_cast<uint64_t *>(out), true, B, s),
                 "pbs lvl02");
         if (post)

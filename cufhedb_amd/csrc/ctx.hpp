@@ -16,6 +16,7 @@ struct fhe_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     fhe::cd *bk1 = nullptr;  // lvl01 key, device FD layout
     fhe::cd *bk2 = nullptr;  // lvl02 key, device FD layout
+    fhe::cd *rot2 = nullptr; // w2^e, e < 4096: the lvl2 rotation in the frequency domain
     uint32_t *ksk[3] = {nullptr, nullptr, nullptr};  // kinds 10, 20, 21
     fhe::cd *trgsw1 = nullptr;  // external-product TRGSW in device layout
     fhe::cd *trgsw2 = nullptr;

@@ -201,7 +201,7 @@ int fhe_compare_digits_dev(fhe_ctx *ctx, int op, const uint64_t *x_dev, int digi
     const uint32_t *ksk20 = ctx->ksk[ksk_slot(20)];
     HIP_TRY(launch_iks(20, x_dev, ksk20, lvl0, n0, int(rows), s, iks_tile(ctx, 20, int(rows))),
             "digit key switch");
-    HIP_TRY(launch_blind_rotate2q(lvl0, n0, ctx->bk2, tv, N2, boot, true, int(rows), s, digits),
+    HIP_TRY(launch_blind_rotate2q(lvl0, n0, ctx->bk2, ctx->rot2, tv, N2, boot, true, int(rows), s, digits),
             "digit bootstrap");
     // 2. the weighted digits summed, minus half a step
     const size_t total = size_t(B) * W2;
@@ -213,7 +213,7 @@ int fhe_compare_digits_dev(fhe_ctx *ctx, int op, const uint64_t *x_dev, int digi
     HIP_TRY(launch_iks(20, sum, ksk20, lvl0, n0, B, s, iks_tile(ctx, 20, B)),
             "combine key switch");
     uint64_t *res = out_level == 2 ? static_cast<uint64_t *>(out_dev) : boot;
-    HIP_TRY(launch_blind_rotate2q(lvl0, n0, ctx->bk2, tv + size_t(digits) * N2, 0, res, true, B, s),
+    HIP_TRY(launch_blind_rotate2q(lvl0, n0, ctx->bk2, ctx->rot2, tv + size_t(digits) * N2, 0, res, true, B, s),
             "combine bootstrap");
     if (post) HIP_TRY(launch_tlwe_add_body<uint64_t>(res, W2, post, B, s), "result offset");
     if (out_level == 1)

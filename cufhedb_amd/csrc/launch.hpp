@@ -49,10 +49,13 @@ hipError_t launch_blind_rotate1_mfma(const uint32_t *in, int n0, const float *bk
 // four-waves-per-ciphertext blind rotation (pbs_lvl2q.hip) and its key layout
 size_t bk2q_device_elems(int n0);
 hipError_t launch_bk2q_from_int(const uint64_t *bk_int, cd *bk_dev, int n0, hipStream_t s);
-// tv_period > 0: ciphertext c uses test vector c % tv_period (tv_stride apart)
-hipError_t launch_blind_rotate2q(const uint32_t *in, int n0, const cd *bk, const uint64_t *tv,
-                                 int tv_stride, uint64_t *out, bool sample_extract, int B,
-                                 hipStream_t s, int tv_period = 0);
+// tv_period > 0: ciphertext c uses test vector c % tv_period (tv_stride apart).
+// rot: the table of launch_rot2q_table (w2^e, e < 4096)
+hipError_t launch_blind_rotate2q(const uint32_t *in, int n0, const cd *bk, const cd *rot,
+                                 const uint64_t *tv, int tv_stride, uint64_t *out,
+                                 bool sample_extract, int B, hipStream_t s, int tv_period = 0);
+size_t rot2q_elems();
+hipError_t launch_rot2q_table(cd *rot, hipStream_t s);
 hipError_t launch_ifft2q(const uint64_t *poly, double *fd, int B, hipStream_t s);
 // external product, TwistFFT and sample extraction (pbs_lvl2.hip)
 hipError_t launch_trgsw2_from_fft(const double *trgswfft, cd *trgsw_dev, hipStream_t s);
