@@ -470,6 +470,17 @@ def main():
             res["mixed_stream"] = {"sequential": seq, "coalesced": coal}
             print(f"bench: mixed stream: {res['mixed_stream']}", file=sys.stderr, flush=True)
         octx.close()
+        if args.mixed:
+            # the same plan at n0 = 636 with the comparisons digit-wise
+            # against public thresholds (the HomMSB chain needs n0 = 672)
+            octx, osk, op = operator_context(F, F.PRESET_CFG636)
+            if "autotune" in res:
+                apply_tuning(F, octx, res["autotune"])
+            _, dig = mixed.run(octx, osk, op, plan, compare="digits")
+            res["mixed_stream"]["sequential_digits_n636"] = dig
+            print(f"bench: mixed stream, digit-wise compares at n0=636: {dig}", file=sys.stderr,
+                  flush=True)
+            octx.close()
         if not args.no_query:
             # the 20-bit comparison workload at its own parameter set, n0 = 636
             octx, osk, op = operator_context(F, F.PRESET_CFG636)

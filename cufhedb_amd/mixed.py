@@ -17,16 +17,21 @@ Results are decrypted and checked after the timed region in both.
 
 Plan (seeded): batch sizes clip(round(lognormal(ln 2048, sigma)), 256, 65536),
 a fraction `frac_lvl1` of the batches are Z_8 LUT PBS (lvl0 -> lvl1), the rest
-are x <= t range compares of 20-bit lvl2 values against one encrypted
-threshold per batch.
+are x <= t range compares of 20-bit lvl2 values: ``compare="chain"`` against
+one encrypted threshold per batch (fhe_compare_dev, the reference's HomMSB
+chain), ``compare="digits"`` against one public threshold per batch with the
+values as five 4-bit lvl2 digits (fhe_compare_digits_dev; one call per
+batch, ``run`` only).
 """
 
 import ctypes
 
 import numpy as np
 
-from . import (CMP_LE, LOGIC, LVL1_N, DeviceBuffer, Scheduler, check, lib, tlwe_encrypt_u32,
-               tlwe_encrypt_u64, tlwe_phase_u32)
+from . import (CMP_LE, LOGIC, LVL1_N, DeviceBuffer, Scheduler, check, encrypt_digits, lib,
+               tlwe_encrypt_u32, tlwe_encrypt_u64, tlwe_phase_u32)
+
+DIGITS, DIGIT_BITS = 5, 4
 
 
 def z8_testvector(lut, N=LVL1_N):
@@ -47,8 +52,8 @@ def make_plan(seed, nbatches=200, sigma=1.0, frac_lvl1=0.7, max_size=65536):
 class _Batch:
     """Device inputs / outputs and the plaintext expectation of one batch."""
 
-    def __init__(self, bi, kind, B, sk, params, rng, bits):
-        self.kind, self.B = kind, B
+    def __init__(self, bi, kind, B, sk, params, rng, bits, compare="chain"):
+        self.kind, self.B, self.compare = kind, B, compare
         if kind == "pbs1":
             m = rng.integers(0, 8, B)
             lut = rng.integers(0, 8, 8)
@@ -57,6 +62,14 @@ class _Batch:
             self.inp = DeviceBuffer.from_array(c0)
             self.aux = DeviceBuffer.from_array(z8_testvector(lut))
             self.expect = lut[m]
+        elif compare == "digits":
+            assert bits <= DIGITS * DIGIT_BITS
+            x = rng.integers(0, 1 << bits, B)
+            self.threshold = int(rng.integers(0, 1 << bits))
+            self.inp = DeviceBuffer.from_array(encrypt_digits(20_000 + bi, 0, sk.s2, x, DIGITS,
+                                                              DIGIT_BITS, params.alpha2))
+            self.aux = None
+            self.expect = x <= self.threshold
         else:
             x = rng.integers(0, 1 << bits, B)
             t = int(rng.integers(0, 1 << bits))
@@ -81,25 +94,32 @@ class _Batch:
 
     @property
     def bootstraps(self):
-        # a 21-bit lvl2 HomMSB chain is 6 lvl2 + 3 lvl1 bootstraps per row
-        return self.B if self.kind == "pbs1" else 9 * self.B
+        # a 21-bit lvl2 HomMSB chain is 6 lvl2 + 3 lvl1 bootstraps per row,
+        # the digit-wise comparison 6 lvl2 (one per digit, one combining)
+        if self.kind == "pbs1":
+            return self.B
+        return (DIGITS + 1) * self.B if self.compare == "digits" else 9 * self.B
 
     def free(self):
         for b in (self.inp, self.aux, self.out):
-            b.free()
+            if b is not None:
+                b.free()
 
 
-def run(ctx, sk, params, plan, seed=5, bits=20):
+def run(ctx, sk, params, plan, seed=5, bits=20, compare="chain"):
     """Each batch one call, timed alone; returns per-batch records and a summary."""
     L = lib()
     rng = np.random.default_rng(seed)
     records = []
     for bi, (kind, B) in enumerate(plan):
-        b = _Batch(bi, kind, B, sk, params, rng, bits)
+        b = _Batch(bi, kind, B, sk, params, rng, bits, compare)
         ctx.sync()
         check(L.fhe_timer_start(ctx.h, None))
         if kind == "pbs1":
             check(L.fhe_pbs_lvl01_dev(ctx.h, b.inp.ptr, b.aux.ptr, 0, b.out.ptr, B, None))
+        elif compare == "digits":
+            check(L.fhe_compare_digits_dev(ctx.h, CMP_LE, b.inp.ptr, DIGITS, DIGIT_BITS,
+                                           b.threshold, 1, b.out.ptr, LOGIC, B, None))
         else:
             check(L.fhe_compare_dev(ctx.h, 2, CMP_LE, b.inp.ptr, b.aux.ptr, 1, b.out.ptr, bits,
                                     LOGIC, B, None))

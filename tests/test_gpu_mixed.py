@@ -20,6 +20,19 @@ def test_mixed_stream_small_plan(full672):
     assert all(r["ms"] > 0 for r in records)
 
 
+@pytest.mark.gpu
+def test_mixed_stream_digit_compares_n636(full636):
+    """The same stream at n0 = 636 with the 20-bit comparisons digit-wise
+    against public thresholds (fhe_compare_digits_dev): every row right."""
+    k, ctx = full636
+    plan = mixed.make_plan(seed=559, nbatches=8, sigma=0.8, frac_lvl1=0.5, max_size=1024)
+    records, summary = mixed.run(ctx, k.sk, k.params, plan, seed=560, compare="digits")
+    print("mixed stream, digit-wise compares:", summary)
+    assert summary["errors"] == 0
+    cmp = [r for r in records if r["kind"] == "cmp2"]
+    assert cmp and all(r["bootstraps"] == 6 * r["size"] for r in cmp)
+
+
 def test_mixed_plan_shape():
     plan = mixed.make_plan(seed=1, nbatches=200)
     sizes = np.array([s for _, s in plan])
