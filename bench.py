@@ -200,6 +200,70 @@ def run_compare_digits(F, ctx, sk, p, rows, reps):
             "rows": rows, "ms_per_query": ms, "errors": errors}
 
 
+def run_groupby(F, ctx, sk, p, rows, reps):
+    """The filter-aggregate query over a seeded synthetic table of `rows`
+    rows: quantity [1, 50], price [900, 105000], discount [0, 10], tax
+    [0, 8], cat1 {0, 1, 2}, cat2 {0, 1}, date [0, 2556]; WHERE date <= 2400
+    GROUP BY cat1, cat2 with SUM(quantity), SUM(price * (100 - discount)),
+    COUNT(*).  The encrypted part (date compared digit-wise on lvl2 against
+    the public cut-off, category predicates, six group ANDs, log_rescale) is
+    timed eagerly and as one replayed HIP graph; the aggregates over the
+    decrypted predicates are compared with the plaintext query."""
+    from cufhedb_amd import queries as Q
+
+    rng = np.random.default_rng(41)
+    cutoff = 2400
+    t = {"quantity": rng.integers(1, 51, rows), "price": rng.integers(900, 105001, rows),
+         "discount": rng.integers(0, 11, rows), "tax": rng.integers(0, 9, rows),
+         "cat1": rng.integers(0, 3, rows), "cat2": rng.integers(0, 2, rows),
+         "date": rng.integers(0, 2557, rows)}
+
+    def enc1(v, b, seed):
+        m = (np.asarray(v, np.uint64) << np.uint64(32 - b - 1)).astype(np.uint32)
+        return F.DeviceBuffer.from_array(F.tlwe_encrypt_u32(seed, 0, sk.s1, m, p.alpha1))
+
+    dd = F.DeviceBuffer.from_array(F.encrypt_digits(341, 0, sk.s2, t["date"], 3, 4, p.alpha2))
+    c1, c2 = enc1(t["cat1"], 2, 342), enc1(t["cat2"], 1, 343)
+    consts = [enc1([1], 2, 344), enc1([2], 2, 345), enc1([1], 1, 346)]
+    scratch = Q.GroupByScratch(rows)
+    run = lambda: Q.groupby_filter(ctx, dd, c1, c2, *consts, rows, cutoff,  # noqa: E731
+                                   scratch=scratch)
+    ok = t["date"] <= cutoff
+    want = np.array([[int(np.sum(t["quantity"][m])),
+                      int(np.sum(t["price"][m] * (100 - t["discount"][m]))), int(np.sum(m))]
+                     for m in (ok & (t["cat1"] == a) & (t["cat2"] == b) for a, b in Q.GROUPS)],
+                    np.int64)
+
+    def errors():
+        pred = Q.decrypt_predicates(scratch.out, rows, sk.s1)
+        e = sum(int(np.sum(pred[g] != (ok & (t["cat1"] == a) & (t["cat2"] == b))))
+                for g, (a, b) in enumerate(Q.GROUPS))
+        agg = Q.group_aggregates(pred, t["quantity"], t["price"], t["discount"])
+        return e, int(np.sum(agg != want))
+
+    ms = timed_on(F, ctx, run, reps)
+    err_pred, err_agg = errors()
+    graph = F.Graph(ctx, run)
+    ms_graph = timed_on(F, ctx, graph.launch, reps)
+    gerr_pred, gerr_agg = errors()
+    graph.destroy()
+    scratch.free()
+    for b in [dd, c1, c2] + consts:
+        b.free()
+    print(f"bench: group-by query {rows} rows, n0={p.n0}: {ms:.1f} ms/query "
+          f"({err_pred} predicate / {err_agg} aggregate errors); graph replay {ms_graph:.1f} ms "
+          f"({gerr_pred} / {gerr_agg})", file=sys.stderr, flush=True)
+    return {"workload": f"SELECT cat1, cat2, SUM(quantity), SUM(price*(100-discount)), COUNT(*) "
+                        f"WHERE date <= {cutoff} GROUP BY cat1, cat2 over {rows} seeded rows, "
+                        f"n0={p.n0}: date as three 4-bit lvl2 digits vs the public cut-off "
+                        "(4 lvl2 PBS), 3 lvl1 category compares, 10 lvl1 ANDs, 6 log_rescales "
+                        "per row; aggregates over the decrypted predicates",
+            "rows": rows, "ms_per_query": ms, "ms_per_query_graph": ms_graph,
+            "predicate_errors": err_pred, "aggregate_errors": err_agg,
+            "predicate_errors_graph": gerr_pred, "aggregate_errors_graph": gerr_agg,
+            "aggregates": {f"{a},{b}": want[g].tolist() for g, (a, b) in enumerate(Q.GROUPS)}}
+
+
 def run_queries(F, ctx, sk, p, rows, reps):
     """Encrypted query pipelines, each timed with HIP events on the context
     stream over `reps` runs after one untimed run (which also sizes the
@@ -487,6 +551,8 @@ def main():
             apply_tuning(F, octx, res["autotune"])
             res["queries"]["compare_digits_n636"] = run_compare_digits(
                 F, octx, osk, op, args.query_rows, args.query_reps)
+            res["queries"]["groupby_n636"] = run_groupby(F, octx, osk, op, args.query_rows,
+                                                         args.query_reps)
             octx.close()
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)

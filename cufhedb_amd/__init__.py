@@ -105,6 +105,8 @@ _SIGS = {
     "fhe_pbs_lvl01_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp]),
     "fhe_blind_rotate_lvl01_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp]),
     "fhe_pbs_lvl02_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp]),
+    "fhe_pbs_many_lut_lvl01_dev": (_i, [_vp, _vp, _vp, _i, _i, _vp, _i, _vp]),
+    "fhe_pbs_many_lut_lvl02_dev": (_i, [_vp, _vp, _vp, _i, _i, _vp, _i, _vp]),
     "fhe_blind_rotate_lvl02_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp]),
     "fhe_sample_extract_lvl1_dev": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
     "fhe_sample_extract_lvl2_dev": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
@@ -116,6 +118,7 @@ _SIGS = {
     "fhe_ifft_lvl2_dev": (_i, [_vp, _vp, _vp, _i, _vp]),
     "fhe_fft_lvl2_dev": (_i, [_vp, _vp, _vp, _i, _vp]),
     "fhe_gate_lvl1_dev": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp]),
+    "fhe_gate_rescale_lvl1_dev": (_i, [_vp, _i, _vp, _vp, _vp, _u32, _i, _vp]),
     "fhe_msb_gate_lvl1_dev": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
     "fhe_ide_gate_lvl1_dev": (_i, [_vp, _vp, _vp, _u32, _i, _vp]),
     "fhe_msb_gate_dev": (_i, [_vp, _i, _i, _vp, _vp, _i, _i, _vp]),
@@ -529,6 +532,27 @@ class Context:
         return self._run_dev(lib().fhe_pbs_lvl02_dev, [tlwe0, tv], (B, LVL2_N + 1), np.uint64,
                              B, stride_first=stride)
 
+    def pbs_many_lut(self, level, tlwe0, tv, num_out):
+        """GateBootstrappingManyLUT: num_out lookup tables from one blind
+        rotation (fhe_pbs_many_lut_lvl0X_dev).  tv: one test vector (N words)
+        or one per ciphertext (B x N), tables interleaved as
+        many_lut_testvector builds them.  Returns B x num_out x (N+1)."""
+        N, dt, fn = ((LVL1_N, np.uint32, lib().fhe_pbs_many_lut_lvl01_dev) if level == 1 else
+                     (LVL2_N, np.uint64, lib().fhe_pbs_many_lut_lvl02_dev))
+        tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)
+        tv = np.ascontiguousarray(tv, dt)
+        B = tlwe0.shape[0]
+        stride = 0 if tv.ndim == 1 else N
+        dins = [DeviceBuffer.from_array(tlwe0), DeviceBuffer.from_array(tv)]
+        dout = DeviceBuffer(max(B, 1) * num_out * (N + 1) * np.dtype(dt).itemsize)
+        try:
+            check(fn(self.h, dins[0].ptr, dins[1].ptr, stride, num_out, dout.ptr, B, None))
+            self.sync()
+            return dout.to_array(dt, (B, num_out, N + 1))
+        finally:
+            for d in dins + [dout]:
+                d.free()
+
     def blind_rotate_lvl02(self, tlwe0, tv):
         tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)
         tv = np.ascontiguousarray(tv, np.uint64)
@@ -690,6 +714,32 @@ class Graph:
             self.destroy()
         except Exception:
             pass
+
+
+def many_lut_testvector(luts, plain_bits, N=LVL1_N):
+    """Test vector of a many-table bootstrap (pbs_many_lut) for messages
+    m * 2^(w - plain_bits - 1) (one padding bit; w = 32 for N = 1024, 64 for
+    N = 2048): luts is num_out x 2^plain_bits torus words, table k answering
+    luts[k][m].  Coefficient j belongs to table j mod 2^bw (bw = bit length of
+    num_out - 1, the stride of BlindRotate<P, num_out>'s modulus switch,
+    thirdparty/TFHEpp/include/gatebootstrapping.hpp:21-41); its box is
+    (base + box/2) // box for base = j - (j mod 2^bw) and box = N / 2^plain_bits,
+    the last half box wrapping negacyclically to -luts[k][0]."""
+    dt = np.uint32 if N == LVL1_N else np.uint64
+    w = 32 if N == LVL1_N else 64
+    luts = [[int(v) for v in t] for t in luts]
+    K = len(luts)
+    stride = 1 << (K - 1).bit_length()
+    box = N >> plain_bits
+    tv = np.zeros(N, dt)
+    for j in range(N):
+        k = j % stride
+        if k >= K:
+            continue
+        q = (j - k + box // 2) // box
+        v = -luts[k][0] if q == 1 << plain_bits else luts[k][q]
+        tv[j] = v % (1 << w)
+    return tv
 
 
 def encrypt_digits(seed, index0, s2, values, digits, digit_bits, alpha):

@@ -133,6 +133,8 @@ int fhe_ctx_destroy(fhe_ctx *ctx)
         if (b) (void)hipFree(b);
     for (uint32_t *b : ctx->digit)
         if (b) (void)hipFree(b);
+    for (uint32_t *b : ctx->many)
+        if (b) (void)hipFree(b);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -501,6 +503,24 @@ int fhe_gate_lvl1_dev(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint3
         set_error("fhe_gate_lvl1_dev: unknown gate %d", gate);
         return FHE_ERR_ARG;
     }
+}
+
+int fhe_gate_rescale_lvl1_dev(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint32_t *b_dev,
+                              uint32_t *out_dev, uint32_t scale_bits, int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && a_dev && b_dev && out_dev && B >= 0,
+                  "fhe_gate_rescale_lvl1_dev: bad argument");
+    FHE_CHECK_ARG(gate == FHE_GATE_AND || gate == FHE_GATE_OR,
+                  "fhe_gate_rescale_lvl1_dev: gate must be AND or OR (got %d)", gate);
+    FHE_CHECK_ARG(scale_bits >= 1 && scale_bits <= 31,
+                  "fhe_gate_rescale_lvl1_dev: scale_bits in [1, 31]");
+    // HomAND / HomOR's combination a + b -+ mu/2, then log_rescale's test
+    // vector m = 2^(scale_bits-1) and post-offset m: true -> 2^scale_bits,
+    // false -> 0
+    const uint32_t half = Lvl1::mu >> 1;
+    const uint32_t m = 1u << (scale_bits - 1);
+    return gate_pipeline(ctx, a_dev, b_dev, 1, 1, gate == FHE_GATE_AND ? 0u - half : half, m,
+                         false, 0, m, out_dev, B, pick(ctx, stream));
 }
 
 int fhe_msb_gate_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_dev, int result_type,

@@ -50,6 +50,9 @@ struct fhe_ctx {
     static constexpr int kDigitBufs = 4;
     uint32_t *digit[kDigitBufs] = {};
     size_t digit_cap[kDigitBufs] = {};
+    // scratch of the many-table bootstrap: rounded lvl0 batch, accumulators
+    uint32_t *many[2] = {};
+    size_t many_cap[2] = {};
     // tuned choices per knob (FHE_TUNE_*) and batch size (ascending batch
     // sizes; a batch uses the entry of the largest size <= it)
     static constexpr int kTuneKnobs = 3;

@@ -162,6 +162,17 @@ int fhe_pbs_lvl02_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint64_t *tv_d
 /* Blind rotation only, lvl2 accumulator (B x 2 x 2048 uint64_t). */
 int fhe_blind_rotate_lvl02_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint64_t *tv_dev,
                                int tv_stride, uint64_t *acc_dev, int B, void *stream);
+/* Programmable bootstrap with num_out lookup tables in one blind rotation
+ * (TFHEpp GateBootstrappingManyLUT<lvl01param / lvl02param, num_out>,
+ * thirdparty/TFHEpp/include/gatebootstrapping.hpp:119-130): the modulus
+ * switch keeps multiples of 2^bw, bw = bits_needed(num_out - 1), so test
+ * vector coefficient j belongs to table j mod 2^bw; output k of a
+ * ciphertext is SampleExtractIndex(acc, k).  num_out in [1, 64].
+ *   out_dev: B x num_out x (N+1) words (num_out TLWEs per ciphertext). */
+int fhe_pbs_many_lut_lvl01_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint32_t *tv_dev,
+                               int tv_stride, int num_out, uint32_t *out_dev, int B, void *stream);
+int fhe_pbs_many_lut_lvl02_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint64_t *tv_dev,
+                               int tv_stride, int num_out, uint64_t *out_dev, int B, void *stream);
 /* SampleExtractIndex (thirdparty/TFHEpp/src/trlwe.cpp:109-118;
  * BlindRotate_gpu.cuh:386-397,703-718): B TRLWEs -> B TLWEs. */
 int fhe_sample_extract_lvl1_dev(fhe_ctx *ctx, const uint32_t *trlwe_dev, uint32_t *tlwe_dev,
@@ -214,6 +225,15 @@ enum fhe_result_type { FHE_ARITHMETIC = 0, FHE_LOGIC = 1 };
  * HEDB/utils/types.h:4-11; NAND/NOR/XNOR are always LOGIC. */
 int fhe_gate_lvl1_dev(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint32_t *b_dev,
                       uint32_t *out_dev, int result_type, int B, void *stream);
+/* AND / OR of two LOGIC lvl1 TLWEs with the result already at the scale
+ * log_rescale gives it: 0 or 2^scale_bits (scale_bits in [1, 31]).  No
+ * reference counterpart as one call: it decrypts as
+ * log_rescale(HomAND / HomOR(a, b), scale_bits) (tfhepp_utils.cpp:28-35 after
+ * gatebootstrap_gpu.cuh:123-153, the pair the query filters run,
+ * test/HE3_Query/TPCH_Q1.cu:266-283) with one bootstrap instead of two: the
+ * gate's linear combination goes straight into the rescaling bootstrap. */
+int fhe_gate_rescale_lvl1_dev(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint32_t *b_dev,
+                              uint32_t *out_dev, uint32_t scale_bits, int B, void *stream);
 /* cuHEDB::MSBGateBootstrapping (lvl1 -> lvl1, gatebootstrap_gpu.cuh:31-46) */
 int fhe_msb_gate_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out_dev,
                           int result_type, int B, void *stream);

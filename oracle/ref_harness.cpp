@@ -26,6 +26,33 @@ const A &as(const void *p)
 }
 }  // namespace
 
+namespace {
+template <class P, uint32_t K>
+void many_lut(typename P::targetP::T *out, const uint32_t *tlwe0, const double *bkfft,
+              const typename P::targetP::T *tv)
+{
+    std::array<TLWE<typename P::targetP>, K> r;
+    GateBootstrappingManyLUT<P, K>(r, as<TLWE<lvl0param>>(tlwe0),
+                                   as<BootstrappingKeyFFT<P>>(bkfft),
+                                   as<Polynomial<typename P::targetP>>(tv));
+    std::memcpy(out, r.data(), sizeof(r));
+}
+
+template <class P>
+int many_lut_n(typename P::targetP::T *out, const uint32_t *tlwe0, const double *bkfft,
+               const typename P::targetP::T *tv, int num_out)
+{
+    switch (num_out) {
+    case 1: many_lut<P, 1>(out, tlwe0, bkfft, tv); return 0;
+    case 2: many_lut<P, 2>(out, tlwe0, bkfft, tv); return 0;
+    case 3: many_lut<P, 3>(out, tlwe0, bkfft, tv); return 0;
+    case 4: many_lut<P, 4>(out, tlwe0, bkfft, tv); return 0;
+    case 8: many_lut<P, 8>(out, tlwe0, bkfft, tv); return 0;
+    default: return -1;
+    }
+}
+}  // namespace
+
 extern "C" {
 
 int ref_lvl0_n() { return lvl0param::n; }
@@ -135,6 +162,21 @@ void ref_pbs_lvl02(uint64_t *out, const uint32_t *tlwe0, const double *bkfft, co
                                               as<BootstrappingKeyFFT<lvl02param>>(bkfft),
                                               as<Polynomial<lvl2param>>(tv));
     std::memcpy(out, r.data(), sizeof(r));
+}
+
+/* GateBootstrappingManyLUT<lvl0Xparam, num_out> for num_out in {1, 2, 3, 4, 8}
+ * (gatebootstrapping.hpp:119-130); out = num_out TLWEs.  Returns -1 for
+ * other counts. */
+int ref_pbs_many_lvl01(uint32_t *out, const uint32_t *tlwe0, const double *bkfft,
+                       const uint32_t *tv, int num_out)
+{
+    return many_lut_n<lvl01param>(out, tlwe0, bkfft, tv, num_out);
+}
+
+int ref_pbs_many_lvl02(uint64_t *out, const uint32_t *tlwe0, const double *bkfft,
+                       const uint64_t *tv, int num_out)
+{
+    return many_lut_n<lvl02param>(out, tlwe0, bkfft, tv, num_out);
 }
 
 void ref_sample_extract_lvl1(uint32_t *tlwe, const uint32_t *trlwe, int index)

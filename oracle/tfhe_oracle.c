@@ -383,6 +383,43 @@ DEF_SE(uint64_t, u64)
 DEF_PBS(uint32_t, u32)
 DEF_PBS(uint64_t, u64)
 
+/* GateBootstrappingManyLUT<P, num_out> (TFHEpp/include/gatebootstrapping.hpp:119-130
+ * with BlindRotate<P, num_out>, :15-43): bw = bits_needed(num_out - 1)
+ * (TFHEpp/include/utils.hpp:69-81, the bit length of num_out - 1), then
+ *   bbar = 2N - ((b >> (32 - 1 - nbit + bw)) << bw)
+ *   abar = ((a + 2^(32 - 2 - nbit + bw)) >> (32 - 1 - nbit + bw)) << bw
+ * and SampleExtractIndex at 0 .. num_out-1 (out: num_out x (N+1) words). */
+static int bits_needed_u32(uint32_t v)
+{
+    int b = 0;
+    while (v >> b) b++;
+    return b;
+}
+
+#define DEF_PBS_MANY(T, SUF)                                                         \
+    void orc_pbs_many_##SUF(T *out, const uint32_t *tlwe0, int n0, const double *bkfft, \
+                            const T *tv, int N, int nbit, int l, int Bgbit, int num_out) \
+    {                                                                                \
+        const int bw = bits_needed_u32((uint32_t)num_out - 1);                       \
+        const int sh = 32 - 1 - nbit + bw;                                           \
+        T *acc = (T *)malloc(sizeof(T) * 2 * N);                                     \
+        uint32_t bbar = 2u * N - ((tlwe0[n0] >> sh) << bw);                          \
+        memset(acc, 0, sizeof(T) * N);                                               \
+        orc_mul_xai_##SUF(acc + N, tv, bbar, N);                                     \
+        const uint32_t roundoffset = 1u << (sh - 1);                                 \
+        const size_t trgswlen = (size_t)2 * l * 2 * N;                               \
+        for (int i = 0; i < n0; i++) {                                               \
+            uint32_t abar = ((tlwe0[i] + roundoffset) >> sh) << bw;                  \
+            if (abar == 0) continue;                                                 \
+            orc_cmux_rot_##SUF(acc, bkfft + (size_t)i * trgswlen, abar, N, l, Bgbit); \
+        }                                                                            \
+        for (int k = 0; k < num_out; k++)                                            \
+            orc_sample_extract_##SUF(out + (size_t)k * (N + 1), acc, k, N);          \
+        free(acc);                                                                   \
+    }
+DEF_PBS_MANY(uint32_t, u32)
+DEF_PBS_MANY(uint64_t, u64)
+
 /* ---- identity key switch ------------------------------------------------- */
 /* out (target, 32-bit torus, tn+1 words) from in (domain, W-bit torus, dn+1). */
 #define DEF_IKS(TD, SUF, W)                                                          \

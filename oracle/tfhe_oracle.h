@@ -94,6 +94,12 @@ void orc_pbs_u32(uint32_t *out, const uint32_t *tlwe0, int n0, const double *bkf
                  const uint32_t *tv, int N, int nbit, int l, int Bgbit);
 void orc_pbs_u64(uint64_t *out, const uint32_t *tlwe0, int n0, const double *bkfft,
                  const uint64_t *tv, int N, int nbit, int l, int Bgbit);
+/* GateBootstrappingManyLUT<P, num_out>: TFHEpp/include/gatebootstrapping.hpp:119-130.
+ * out = num_out TLWEs of N+1 words (sample extraction at 0 .. num_out-1). */
+void orc_pbs_many_u32(uint32_t *out, const uint32_t *tlwe0, int n0, const double *bkfft,
+                      const uint32_t *tv, int N, int nbit, int l, int Bgbit, int num_out);
+void orc_pbs_many_u64(uint64_t *out, const uint32_t *tlwe0, int n0, const double *bkfft,
+                      const uint64_t *tv, int N, int nbit, int l, int Bgbit, int num_out);
 
 /* ---- identity key switching --------------------------------------------- */
 /* IdentityKeySwitch: TFHEpp/src/keyswitch.cpp:12-48, target torus 32-bit. */

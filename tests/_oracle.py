@@ -165,6 +165,28 @@ def pbs32(tlwe0, bkfft, tv, nbit=10, l=3, Bgbit=6):
     return out
 
 
+def pbs_many32(tlwe0, bkfft, tv, num_out, nbit=10, l=3, Bgbit=6):
+    """GateBootstrappingManyLUT<lvl01param, num_out>: num_out x (N+1) words."""
+    tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)
+    N = tv.shape[0]
+    out = np.zeros((num_out, N + 1), np.uint32)
+    tv = np.ascontiguousarray(tv, np.uint32)
+    orc().orc_pbs_many_u32(P(out), P(tlwe0), tlwe0.shape[0] - 1, P(bkfft), P(tv), N, nbit, l,
+                           Bgbit, num_out)
+    return out
+
+
+def pbs_many64(tlwe0, bkfft, tv, num_out, nbit=11, l=4, Bgbit=9):
+    """GateBootstrappingManyLUT<lvl02param, num_out>: num_out x (N+1) words."""
+    tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)
+    N = tv.shape[0]
+    out = np.zeros((num_out, N + 1), np.uint64)
+    tv = np.ascontiguousarray(tv, np.uint64)
+    orc().orc_pbs_many_u64(P(out), P(tlwe0), tlwe0.shape[0] - 1, P(bkfft), P(tv), N, nbit, l,
+                           Bgbit, num_out)
+    return out
+
+
 def sample_extract32(trlwe, index=0):
     trlwe = np.ascontiguousarray(trlwe, np.uint32)
     N = trlwe.shape[-1]

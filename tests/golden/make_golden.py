@@ -12,7 +12,8 @@ integer-exact IKS vectors are uniform words defined row by row from a seed
 TFHEpp is compiled with params/128bit.hpp, so the lvl0 dimension of every
 vector here is n0 = 672.
 
-Usage:  python tests/golden/make_golden.py
+Usage:  python tests/golden/make_golden.py [many_lut]
+        (many_lut: write only pbs_many_lvl01_ref672.npz)
 """
 
 import mmap
@@ -214,8 +215,48 @@ def main():
                         msb_logic=F.tlwe_phase_u32(msb_logic, sk.s1),
                         msb_arith=F.tlwe_phase_u32(msb_arith, sk.s1),
                         ide=F.tlwe_phase_u32(ide, sk.s1))
+    many_lut(ref, sk, p672, bkfft)
     print("golden vectors written to", HERE)
 
 
+def many_lut(ref, sk, p672, bkfft):
+    """GateBootstrappingManyLUT<lvl01param, K> (gatebootstrapping.hpp:119-130)
+    for K = 2 (Z_8 messages) and K = 4 (Z_4 messages: the stride-4 modulus
+    switch quadruples the rounding noise), random tables."""
+    P = O.P
+    rng = np.random.default_rng(777)
+    vec = {"sk_seed": 672001, "bk_seed": 672002}
+    for K, bits in ((2, 3), (4, 2)):
+        B = 6
+        m = rng.integers(0, 1 << bits, B)
+        luts = rng.integers(0, 16, (K, 1 << bits))
+        tv = F.many_lut_testvector(luts << 28, bits, N1)
+        c0 = F.tlwe_encrypt_u32(672300 + K, 0, sk.s0, (m << (31 - bits)).astype(np.uint32),
+                                p672.alpha0)
+        out = np.zeros((B, K, N1 + 1), np.uint32)
+        for b in range(B):
+            assert ref.ref_pbs_many_lvl01(P(out[b]), P(c0[b]), P(bkfft), P(tv), K) == 0
+        vec.update({f"m{K}": m, f"luts{K}": luts, f"tv{K}": tv, f"c0_{K}": c0, f"out{K}": out,
+                    f"phase{K}": F.tlwe_phase_u32(out.reshape(-1, N1 + 1), sk.s1).reshape(B, K)})
+    np.savez_compressed(os.path.join(HERE, "pbs_many_lvl01_ref672.npz"), **vec)
+
+
+def many_lut_only():
+    """Write only the many-table vectors (the other files are unchanged)."""
+    ref = O.ref()
+    if ref is None:
+        sys.exit("oracle/_ref/libtfhepp_ref.so is missing: run `make ref` first")
+    p672 = F.preset(F.PRESET_REF672)
+    sk = F.SecretKeys(p672, 672001)
+    bk = F.keygen_bk_lvl01(sk, 672002)
+    bkfft = np.zeros(bk.shape)
+    for i in range(bk.shape[0]):
+        ref.ref_trgsw_to_fft_lvl1(O.P(bkfft[i]), O.P(np.ascontiguousarray(bk[i])))
+    many_lut(ref, sk, p672, bkfft)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["many_lut"]:
+        many_lut_only()
+    else:
+        main()

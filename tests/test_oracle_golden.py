@@ -112,6 +112,47 @@ def test_golden_pbs_lvl01_ref672():
         assert np.array_equal(O.sample_extract32(g["acc"][b]), g["out"][b])
 
 
+def test_golden_pbs_many_lut_lvl01_ref672():
+    """GateBootstrappingManyLUT<lvl01param, K> of the compiled reference for
+    K = 2 and 4 against the oracle: every output within PHASE_TOL of the
+    reference's phase and decrypting to its table entry."""
+    g = load("pbs_many_lvl01_ref672.npz")
+    sk = F.SecretKeys(F.preset(F.PRESET_REF672), int(g["sk_seed"]))
+    bkfft = O.bk_to_fft32(F.keygen_bk_lvl01(sk, int(g["bk_seed"])))
+    for K in (2, 4):
+        m, luts = g[f"m{K}"], g[f"luts{K}"]
+        for b in range(3):
+            out = O.pbs_many32(g[f"c0_{K}"][b], bkfft, g[f"tv{K}"], K)
+            ph = F.tlwe_phase_u32(out, sk.s1)
+            for k in range(K):
+                assert abs(int(O.tdiff32(ph[k], g[f"phase{K}"][b, k]))) <= PHASE_TOL, (K, b, k)
+                assert int(np.round(O.tdiff32(ph[k], 0) / (1 << 28))) % 16 == luts[k][m[b]]
+
+
+def test_many_lut_is_rounded_single_lut():
+    """The identity the device path is built on: the K-table bootstrap's
+    accumulator equals the single-table blind rotation of the lvl0 words
+    rounded to multiples of 2^sh (sh = 32 - 1 - nbit + bw), and K = 1 is the
+    plain PBS.  Checked on the oracle, bit for bit."""
+    p = F.preset(F.PRESET_CFG636)
+    sk = F.SecretKeys(p, 61)
+    bkfft = O.bk_to_fft32(F.keygen_bk_lvl01(sk, 62))
+    rng = np.random.default_rng(63)
+    c0 = rng.integers(0, 1 << 32, (2, p.n0 + 1), dtype=np.uint64).astype(np.uint32)
+    tv = rng.integers(0, 1 << 32, 1024, dtype=np.uint64).astype(np.uint32)
+    assert np.array_equal(O.pbs_many32(c0[0], bkfft, tv, 1)[0], O.pbs32(c0[0], bkfft, tv))
+    for K, bw in ((2, 1), (3, 2), (8, 3)):
+        sh = 21 + bw
+        r = c0[1].astype(np.uint64)
+        a = (((r[:-1] + (1 << (sh - 1))) % (1 << 32)) >> sh) << sh
+        b = (r[-1:] >> sh) << sh
+        rounded = np.concatenate([a, b]).astype(np.uint32)
+        acc = O.blind_rotate32(rounded, bkfft, tv)
+        many = O.pbs_many32(c0[1], bkfft, tv, K)
+        for k in range(K):
+            assert np.array_equal(many[k], O.sample_extract32(acc, k)), (K, k)
+
+
 def test_golden_iks_bit_exact():
     g = load("iks.npz")
     for b in range(2):

@@ -73,3 +73,21 @@ def test_iks_lvl10_vs_reference_real_key(keys672, rng):
         r = np.zeros(673, np.uint32)
         ref.ref_iks_lvl10(O.P(r), O.P(c1[b]), O.P(k.ksk10))
         assert np.array_equal(O.iks32(c1[b], k.ksk10, 672), r)
+
+
+def test_many_lut_vs_reference(keys672, rng):
+    """GateBootstrappingManyLUT<lvl01param, K> for K = 3 and 8 (tables of a
+    non-power-of-two count, and the widest stride the harness instantiates)
+    against the oracle on fresh ciphertexts: phases within 2^-6 of the torus."""
+    k = keys672
+    for K, bits in ((3, 2), (8, 1)):
+        luts = rng.integers(0, 16, (K, 1 << bits))
+        tv = F.many_lut_testvector(luts << 28, bits)
+        m = rng.integers(0, 1 << bits, 2)
+        c0 = k.enc_lvl0((m << (31 - bits)).astype(np.uint32), index0=900 + K)
+        for b in range(2):
+            r = np.zeros((K, N1 + 1), np.uint32)
+            assert ref.ref_pbs_many_lvl01(O.P(r), O.P(c0[b]), O.P(k.bkfft), O.P(tv), K) == 0
+            got = O.pbs_many32(c0[b], k.bkfft, tv, K)
+            d = O.tdiff32(F.tlwe_phase_u32(got, k.sk.s1), F.tlwe_phase_u32(r, k.sk.s1))
+            assert np.max(np.abs(d)) <= 1 << 26, (K, b)
