@@ -282,21 +282,73 @@ __device__ __forceinline__ void twiddle4(cd v[4], cd base)
     v[3] = CONJ ? cmulc(v[3], b3) : cmul(v[3], b3);
 }
 
-// 64-point DFT over the lanes of a wave for each of the four register
-// sequences, as three radix-4 passes through a wave-private LDS tile of
-// kLaneTile double2 (row strides 64, 20 and 5, conflict-free for 16-lane
-// b128 accesses).  t64 = W64^(lane & 15), t16 = W16^(lane & 3).  Input:
-// register r, lane L holds x_r[L]; output: register q, lane l holds
-// X_r[m] for r = l >> 4, m = ((l >> 2) & 3) + 4 (l & 3) + 16 q.
-__device__ __forceinline__ void lanes64_fwd(cd v[4], cd *tile, int l, cd t64, cd t16)
+// Exchange of one 32-bit word between two registers across lane groups:
+// permlane32_swap trades lanes 32-63 of a with lanes 0-31 of b,
+// permlane16_swap the odd 16-lane rows of a with the even rows of b
+// (gfx950 v_permlane32_swap / v_permlane16_swap; no LDS traffic).
+template <int ROWS>
+__device__ __forceinline__ void swap_rows(double &a, double &b)
 {
-    // T1: (lane L, reg m1) -> lane (L0 = l & 15, m1 = l >> 4), reg L1
+    const uint64_t ua = uint64_t(__double_as_longlong(a)), ub = uint64_t(__double_as_longlong(b));
+    const uint32_t al = uint32_t(ua), ah = uint32_t(ua >> 32);
+    const uint32_t bl = uint32_t(ub), bh = uint32_t(ub >> 32);
+    const auto lo = ROWS == 32 ? __builtin_amdgcn_permlane32_swap(al, bl, false, false)
+                               : __builtin_amdgcn_permlane16_swap(al, bl, false, false);
+    const auto hi = ROWS == 32 ? __builtin_amdgcn_permlane32_swap(ah, bh, false, false)
+                               : __builtin_amdgcn_permlane16_swap(ah, bh, false, false);
+    a = __longlong_as_double((long long)((uint64_t(hi[0]) << 32) | lo[0]));
+    b = __longlong_as_double((long long)((uint64_t(hi[1]) << 32) | lo[1]));
+}
+
+// 4 x 4 transpose of the register index with lane bits 4-5: afterwards
+// register r, lane (l & 15) + 16 m holds what register m, lane
+// (l & 15) + 16 r held.  Register bit 1 trades with lane bit 5 (32-lane
+// swap between registers r and r + 2), register bit 0 with lane bit 4
+// (16-lane swap between registers r and r + 1).  Its own inverse.
+__device__ __forceinline__ void transpose_reg_lane_hi(cd v[4])
+{
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        swap_rows<32>(v[r].x, v[r + 2].x);
+        swap_rows<32>(v[r].y, v[r + 2].y);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r += 2) {
+        swap_rows<16>(v[r].x, v[r + 1].x);
+        swap_rows<16>(v[r].y, v[r + 1].y);
+    }
+}
+
+// The same transpose through the wave's LDS tile (rows of 64 double2).
+__device__ __forceinline__ void transpose_reg_lane_hi_lds(cd v[4], cd *tile, int l)
+{
 #pragma unroll
     for (int m1 = 0; m1 < 4; m1++) tile[m1 * 64 + l] = v[m1];
     wave_lds_sync();
 #pragma unroll
     for (int L1 = 0; L1 < 4; L1++) v[L1] = tile[(l >> 4) * 64 + (l & 15) + 16 * L1];
     wave_lds_sync();
+}
+
+// 64-point DFT over the lanes of a wave for each of the four register
+// sequences, as three radix-4 passes: the first exchange is the register /
+// lane-bits-4-5 transpose (PERM: lane swaps; otherwise through the tile),
+// the other two go through a wave-private LDS tile of kLaneTile double2
+// (row strides 20 and 5, conflict-free for 16-lane b128 accesses).  The
+// swaps move VALU work in for LDS traffic out: the four-wave lvl2 kernel,
+// which is the more LDS-bound, gains 2.5 %; the two-wave lvl1 kernel is
+// within +-4 % either way and keeps the tile.  t64 = W64^(lane & 15),
+// t16 = W16^(lane & 3).  Input: register r, lane L holds x_r[L]; output:
+// register q, lane l holds X_r[m] for r = l >> 4,
+// m = ((l >> 2) & 3) + 4 (l & 3) + 16 q.
+template <bool PERM>
+__device__ __forceinline__ void lanes64_fwd(cd v[4], cd *tile, int l, cd t64, cd t16)
+{
+    // T1: (lane L, reg m1) -> lane (L0 = l & 15, m1 = l >> 4), reg L1
+    if constexpr (PERM)
+        transpose_reg_lane_hi(v);
+    else
+        transpose_reg_lane_hi_lds(v, tile, l);
     dft4<1>(v[0], v[1], v[2], v[3]);
     twiddle4<false>(v, t64);
     // T2: (lane (L0, m1), reg q1) -> lane (L00 = l & 3, rest = l >> 2 = 4 m1 + q1), reg L01
@@ -318,6 +370,7 @@ __device__ __forceinline__ void lanes64_fwd(cd v[4], cd *tile, int l, cd t64, cd
     dft4<1>(v[0], v[1], v[2], v[3]);
 }
 
+template <bool PERM>
 __device__ __forceinline__ void lanes64_inv(cd v[4], cd *tile, int l, cd t64, cd t16)
 {
     dft4<-1>(v[0], v[1], v[2], v[3]);
@@ -337,12 +390,10 @@ __device__ __forceinline__ void lanes64_inv(cd v[4], cd *tile, int l, cd t64, cd
     wave_lds_sync();
     twiddle4<true>(v, t64);
     dft4<-1>(v[0], v[1], v[2], v[3]);
-#pragma unroll
-    for (int L1 = 0; L1 < 4; L1++) tile[(l >> 4) * 64 + (l & 15) + 16 * L1] = v[L1];
-    wave_lds_sync();
-#pragma unroll
-    for (int m1 = 0; m1 < 4; m1++) v[m1] = tile[m1 * 64 + l];
-    wave_lds_sync();
+    if constexpr (PERM)
+        transpose_reg_lane_hi(v);
+    else
+        transpose_reg_lane_hi_lds(v, tile, l);
 }
 
 // frequency index held by (lane s, register q) in the permuted layout

@@ -92,13 +92,13 @@ __device__ __forceinline__ void fft512w_fwd(cd v[4], cd *xch, int lane, int wave
         v[2 * s] = cadd(a, b);
         v[2 * s + 1] = cmul(csub(a, b), tw.t128);
     }
-    lanes64_fwd(v, xch + wave * kLaneTile, lane, tw.t64, tw.t16);
+    lanes64_fwd<false>(v, xch + wave * kLaneTile, lane, tw.t64, tw.t16);
 }
 
 // Inverse (unnormalised: returns 512 z): frequency layout in, time layout out.
 __device__ __forceinline__ void fft512w_inv(cd v[4], cd *xch, int lane, int wave, const TwW &tw)
 {
-    lanes64_inv(v, xch + wave * kLaneTile, lane, tw.t64, tw.t16);
+    lanes64_inv<false>(v, xch + wave * kLaneTile, lane, tw.t64, tw.t16);
 #pragma unroll
     for (int s = 0; s < 2; s++) {
         const cd a = v[2 * s], b = cmulc(v[2 * s + 1], tw.t128);

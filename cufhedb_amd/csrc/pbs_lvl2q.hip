@@ -15,9 +15,11 @@
 //           c_p = w2^{256 p},  n0 < 256,  k < 4;
 //   step C  (wave k): the plain 256-point DFT of u_k, Z[4 m' + k]:
 //           a radix-4 pass over the registers, then the 64-point DFT over
-//           the lanes as three radix-4 passes with three wave-local LDS
-//           transposes (row strides 64, 20 and 5 double2, conflict-free for
-//           gfx950's 16-lanes-per-cycle b128 accesses).
+//           the lanes as three radix-4 passes: the first exchange by
+//           cross-lane register swaps (v_permlane32_swap /
+//           v_permlane16_swap), the other two by wave-local LDS transposes
+//           (row strides 20 and 5 double2, conflict-free for gfx950's
+//           16-lanes-per-cycle b128 accesses).
 // Layouts: time domain wave w, lane L, register p holds z[L + 64 w + 256 p];
 // frequency domain wave k, lane l, register q holds Z[4 m' + k] with
 // m' = (l >> 4) + 4 ((l >> 2) & 3) + 16 (l & 3) + 64 q.  The inverse runs
@@ -102,12 +104,12 @@ __device__ __forceinline__ void fft1024q_fwd(cd v[4], cd *xch, int lane, int wav
     // step C: radix-4 over the registers (stride 64), twiddle W256^{L m1}
     dft4<1>(v[0], v[1], v[2], v[3]);
     twiddle4<false>(v, tw.t256);
-    lanes64_fwd(v, xch + wave * kTileQ, lane, tw.t64, tw.t16);
+    lanes64_fwd<true>(v, xch + wave * kTileQ, lane, tw.t64, tw.t16);
 }
 
 __device__ __forceinline__ void fft1024q_inv(cd v[4], cd *xch, int lane, int wave, const TwQ &tw)
 {
-    lanes64_inv(v, xch + wave * kTileQ, lane, tw.t64, tw.t16);
+    lanes64_inv<true>(v, xch + wave * kTileQ, lane, tw.t64, tw.t16);
     twiddle4<true>(v, tw.t256);
     dft4<-1>(v[0], v[1], v[2], v[3]);
     // all waves finished their tiles before the exchange overwrites them

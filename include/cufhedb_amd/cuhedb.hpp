@@ -283,6 +283,32 @@ class CuFFT_Torus {
         cufhedb_amd::check(fhe_ctx_sync(rt_.ctx()), "fhe_ctx_sync");
         dout_.download(out, out_b);
     }
+    // GateBootstrappingManyLUT<P, num_out> (TFHEpp gatebootstrapping.hpp:119-130):
+    // B lvl0 TLWEs -> B x num_out TLWEs, tables interleaved in tv
+    void pbs_many_lut_batch(T *out, const uint32_t *in, const T *tv, int tv_stride, int num_out,
+                            int B)
+    {
+        const size_t in_b = size_t(B) * (rt_.n0() + 1) * 4;
+        const size_t out_b = size_t(B) * num_out * (P::n + 1) * sizeof(T);
+        const size_t tv_b = (tv_stride ? size_t(B) : 1) * P::n * sizeof(T);
+        din_.reset(in_b);
+        dtv_.reset(tv_b);
+        dout_.reset(out_b);
+        din_.upload(in, in_b);
+        dtv_.upload(tv, tv_b);
+        if constexpr (kLevel == 1)
+            cufhedb_amd::check(fhe_pbs_many_lut_lvl01_dev(rt_.ctx(), din_.as<uint32_t>(),
+                                                          dtv_.as<uint32_t>(), tv_stride, num_out,
+                                                          dout_.as<uint32_t>(), B, nullptr),
+                               "fhe_pbs_many_lut_lvl01_dev");
+        else
+            cufhedb_amd::check(fhe_pbs_many_lut_lvl02_dev(rt_.ctx(), din_.as<uint32_t>(),
+                                                          dtv_.as<uint64_t>(), tv_stride, num_out,
+                                                          dout_.as<uint64_t>(), B, nullptr),
+                               "fhe_pbs_many_lut_lvl02_dev");
+        cufhedb_amd::check(fhe_ctx_sync(rt_.ctx()), "fhe_ctx_sync");
+        dout_.download(out, out_b);
+    }
     void external_product_batch(T *out, const T *trlwe, const double *trgswfft, int B)
     {
         const size_t io_b = size_t(B) * 2 * P::n * sizeof(T);
@@ -412,6 +438,23 @@ inline void HomOR_batch(cufhedb_amd::Runtime &rt, uint32_t *res, const uint32_t 
                         const uint32_t *cb, bool result_type, int B)
 {
     detail::run_gate(rt, FHE_GATE_OR, res, ca, cb, result_type, B);
+}
+// HomAND followed by log_rescale(scale_bits) in one bootstrap
+// (fhe_gate_rescale_lvl1_dev; the pair TPCH_Q1.cu:266-283 runs): results
+// decrypt to 0 / 2^scale_bits
+inline void HomAND_rescale_batch(cufhedb_amd::Runtime &rt, uint32_t *res, const uint32_t *ca,
+                                 const uint32_t *cb, uint32_t scale_bits, int B)
+{
+    const size_t bytes = size_t(B) * (cufhedb_amd::lvl1param::n + 1) * 4;
+    cufhedb_amd::DeviceBuffer da(bytes), db(bytes), dout(bytes);
+    da.upload(ca, bytes);
+    db.upload(cb, bytes);
+    cufhedb_amd::check(fhe_gate_rescale_lvl1_dev(rt.ctx(), FHE_GATE_AND, da.as<uint32_t>(),
+                                                 db.as<uint32_t>(), dout.as<uint32_t>(),
+                                                 scale_bits, B, nullptr),
+                       "fhe_gate_rescale_lvl1_dev");
+    cufhedb_amd::check(fhe_ctx_sync(rt.ctx()), "fhe_ctx_sync");
+    dout.download(res, bytes);
 }
 inline void MSBGateBootstrapping_batch(cufhedb_amd::Runtime &rt, uint32_t *res, const uint32_t *in,
                                        bool result_type, int B)

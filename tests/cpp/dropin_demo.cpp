@@ -146,6 +146,50 @@ int main()
         cuHEDB::equal(rt, cx, cy, r, bits, cuHEDB::LOGIC, 0);
         bad += (int32_t(phase(r.data(), s1.data(), 1024)) > 0) != (x == y);
     }
+    // CuFFT_Torus::pbs_many_lut_batch (GateBootstrappingManyLUT, 2 tables
+    // over Z_8 with a padding bit): table k of ciphertext i gives luts[k][m_i]
+    {
+        const int B = 16, K = 2;
+        uint32_t luts[K][8];
+        for (auto &t : luts)
+            for (auto &v : t) v = uint32_t(gen() % 16) << 28;
+        std::vector<uint32_t> tv(1024);
+        for (int j = 0; j < 1024; j++) {
+            const int k = j % K, q = (j - k + 64) / 128;
+            tv[j] = q == 8 ? 0u - luts[k][0] : luts[k][q];
+        }
+        std::vector<uint32_t> mm(B), c0(size_t(B) * (n0 + 1)), r(size_t(B) * K * 1025);
+        for (auto &v : mm) v = uint32_t(gen() % 8) << 28;
+        cufhedb_amd::check(fhe_tlwe_encrypt_u32(20, 0, s0.data(), n0, mm.data(), p.alpha0,
+                                                c0.data(), B), "enc0");
+        cufftlvl1.pbs_many_lut_batch(r.data(), c0.data(), tv.data(), 0, K, B);
+        for (int i = 0; i < B; i++)
+            for (int k = 0; k < K; k++) {
+                const int32_t d = int32_t(phase(r.data() + (size_t(i) * K + k) * 1025, s1.data(),
+                                                1024) - luts[k][mm[i] >> 28]);
+                bad += d > (1 << 26) || d < -(1 << 26);
+            }
+    }
+    // cuHEDB::HomAND_rescale_batch: AND of LOGIC bits, decrypting to 0 / 2^29
+    {
+        const int B = 8;
+        std::vector<uint32_t> ma(B), mb(B), ca(size_t(B) * 1025), cb(size_t(B) * 1025), r(ca.size());
+        const uint32_t mu = 1u << 29;
+        for (int i = 0; i < B; i++) {
+            ma[i] = (i & 1) ? mu : 0u - mu;
+            mb[i] = (i & 2) ? mu : 0u - mu;
+        }
+        cufhedb_amd::check(fhe_tlwe_encrypt_u32(21, 0, s1.data(), 1024, ma.data(), p.alpha1,
+                                                ca.data(), B), "enc a");
+        cufhedb_amd::check(fhe_tlwe_encrypt_u32(22, 0, s1.data(), 1024, mb.data(), p.alpha1,
+                                                cb.data(), B), "enc b");
+        cuHEDB::HomAND_rescale_batch(rt, r.data(), ca.data(), cb.data(), 29, B);
+        for (int i = 0; i < B; i++) {
+            const uint32_t want = (i & 1) && (i & 2) ? 1u << 29 : 0u;
+            const int32_t d = int32_t(phase(r.data() + size_t(i) * 1025, s1.data(), 1024) - want);
+            bad += d > (1 << 26) || d < -(1 << 26);
+        }
+    }
     // cuHEDB::compare_digits_batch: 12-bit values as three 4-bit lvl2
     // digits against a public threshold (lvl02 key and KSK 20 / 21 made here)
     {
