@@ -15,6 +15,7 @@ this host's cores.
 
 import argparse
 import concurrent.futures as cf
+import ctypes
 import json
 import os
 import sys
@@ -163,8 +164,6 @@ def apply_tuning(F, ctx, tuned):
 def timed_on(F, ctx, fn, reps):
     """Mean milliseconds of fn over `reps` runs after one untimed run (which
     also sizes the operator scratch), HIP events on the context stream."""
-    import ctypes
-
     L_ = F.lib()
     fn()
     ctx.sync()
@@ -494,6 +493,15 @@ def main():
                    "VALU work, priced at the FP64 peak)"),
         "flops_per_pbs": fl,
     }
+    # the same kernel against ceilings measured on this device (FP64 FMA
+    # micro-kernel; streaming copy), next to the published peak above
+    fp64_meas, copy_meas = ctypes.c_double(), ctypes.c_double()
+    F.check(L_.fhe_measure_peaks(ctx.h, ctypes.byref(fp64_meas), ctypes.byref(copy_meas)))
+    roofline["measured"] = {"fp64_fma_tflops": fp64_meas.value,
+                            "frac": achieved / fp64_meas.value,
+                            "copy_gbs": copy_meas.value}
+    print(f"bench: measured FP64 FMA {fp64_meas.value:.1f} TFLOP/s, copy "
+          f"{copy_meas.value:.0f} GB/s", file=sys.stderr, flush=True)
     res = {
         "metric": METRIC,
         "value": value,

@@ -467,6 +467,12 @@ int fhe_device_sync(void);
 int fhe_timer_start(fhe_ctx *ctx, void *stream);
 int fhe_timer_stop(fhe_ctx *ctx, void *stream, float *ms);
 int fhe_device_name(char *buf, int len);
+/* Measured ceilings (no reference counterpart; roofline denominators next
+ * to the published ones): FP64 FMA rate of the vector ALUs in TFLOP/s (16
+ * independent chains per lane, 8 workgroups of 4 waves per CU) and the
+ * streaming copy bandwidth in GB/s (512 MiB, read + write bytes).  Uses the
+ * context stream; allocates and frees its own buffers.  Synchronous. */
+int fhe_measure_peaks(fhe_ctx *ctx, double *fp64_tflops, double *copy_gbs);
 
 #ifdef __cplusplus
 }
