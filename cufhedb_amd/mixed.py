@@ -92,6 +92,24 @@ class _Batch:
             return int(np.sum(np.round(ph / (1 << 28)).astype(np.int64) % 16 != self.expect))
         return int(np.sum((ph > 0) != self.expect))
 
+    def phase_errors(self, sk):
+        """Signed distance (torus units of 2^-32) of every output phase from
+        the exact encoding of the expected result: LUT value * 2^28 for the
+        lvl1 PBS (decision margin 2^27), +-2^29 for the comparisons (margin
+        2^29)."""
+        ph = tlwe_phase_u32(self.result(), sk.s1).astype(np.int64)
+        if self.kind == "pbs1":
+            want = self.expect.astype(np.int64) << 28
+        else:
+            want = np.where(self.expect, 1 << 29, -(1 << 29))
+        return (ph - want + (1 << 31)) % (1 << 32) - (1 << 31)
+
+    def record(self, sk, ms):
+        e = self.phase_errors(sk).astype(np.float64)
+        return {"kind": self.kind, "size": self.B, "ms": float(ms), "errors": self.errors(sk),
+                "bootstraps": self.bootstraps, "err_max": float(np.max(np.abs(e))),
+                "err_ss": float(np.sum(e * e))}
+
     @property
     def bootstraps(self):
         # a 21-bit lvl2 HomMSB chain is 6 lvl2 + 3 lvl1 bootstraps per row,
@@ -125,8 +143,7 @@ def run(ctx, sk, params, plan, seed=5, bits=20, compare="chain"):
                                     LOGIC, B, None))
         ms = ctypes.c_float()
         check(L.fhe_timer_stop(ctx.h, None, ctypes.byref(ms)))
-        records.append({"kind": kind, "size": B, "ms": float(ms.value), "errors": b.errors(sk),
-                        "bootstraps": b.bootstraps})
+        records.append(b.record(sk, ms.value))
         b.free()
     return records, summarize(records)
 
@@ -148,8 +165,7 @@ def run_coalesced(ctx, sk, params, plan, seed=5, bits=20, max_rows=16384):
     sched.flush()
     lat = [sched.wait(t) for t in tickets]
     sched.close()
-    records = [{"kind": b.kind, "size": b.B, "ms": ms, "errors": b.errors(sk),
-                "bootstraps": b.bootstraps} for b, ms in zip(batches, lat)]
+    records = [b.record(sk, ms) for b, ms in zip(batches, lat)]
     for b in batches:
         b.free()
     out = summarize(records, span_ms=max(lat) if lat else 0.0)
@@ -178,6 +194,12 @@ def summarize(records, span_ms=None):
         entry = {"batches": len(rs), "rows": rows,
                  "p50_batch_ms": float(np.percentile([r["ms"] for r in rs], 50)),
                  "p99_batch_ms": float(np.percentile([r["ms"] for r in rs], 99))}
+        if all("err_max" in r for r in rs):
+            # output phase error against the exact encoding, over every row
+            entry["phase_err_max_log2"] = float(np.log2(max(r["err_max"] for r in rs) + 1))
+            entry["phase_err_std_log2"] = float(
+                0.5 * np.log2(sum(r["err_ss"] for r in rs) / rows + 1))
+            entry["decision_margin_log2"] = 27.0 if kind == "pbs1" else 29.0
         if span_ms is None:
             ms = sum(r["ms"] for r in rs)
             entry["device_ms"] = ms

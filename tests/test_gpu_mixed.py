@@ -18,6 +18,12 @@ def test_mixed_stream_small_plan(full672):
     assert summary["errors"] == 0
     assert len(records) == len(plan)
     assert all(r["ms"] > 0 for r in records)
+    for kind in ("pbs1", "cmp2"):
+        if kind in summary:
+            e = summary[kind]
+            # every output phase within its decision margin, with room to spare
+            assert e["phase_err_max_log2"] < e["decision_margin_log2"] - 1, (kind, e)
+            assert e["phase_err_std_log2"] < e["phase_err_max_log2"]
 
 
 @pytest.mark.gpu
@@ -118,3 +124,22 @@ def test_scheduler_failed_flush_fails_its_requests(keys636):
     for b in (inp, tv, out):
         b.free()
     ctx.close()
+
+
+def test_summary_error_bounds():
+    """summarize: per-kind maximum and RMS phase error (log2) over every row
+    of every batch, next to the decision margin."""
+    recs = [{"kind": "pbs1", "size": 4, "ms": 1.0, "errors": 0, "bootstraps": 4,
+             "err_max": 2.0 ** 20, "err_ss": 4 * 2.0 ** 36},
+            {"kind": "pbs1", "size": 12, "ms": 3.0, "errors": 0, "bootstraps": 12,
+             "err_max": 2.0 ** 22, "err_ss": 12 * 2.0 ** 36},
+            {"kind": "cmp2", "size": 8, "ms": 2.0, "errors": 0, "bootstraps": 72,
+             "err_max": 2.0 ** 24 - 1, "err_ss": 8 * 2.0 ** 40}]
+    out = mixed.summarize(recs)
+    assert abs(out["pbs1"]["phase_err_max_log2"] - 22) < 1e-6
+    assert abs(out["pbs1"]["phase_err_std_log2"] - 18) < 1e-6
+    assert out["pbs1"]["decision_margin_log2"] == 27.0
+    assert abs(out["cmp2"]["phase_err_max_log2"] - 24) < 1e-6
+    assert abs(out["cmp2"]["phase_err_std_log2"] - 20) < 1e-6
+    assert out["cmp2"]["decision_margin_log2"] == 29.0
+    assert out["errors"] == 0 and out["batches"] == 3
