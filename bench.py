@@ -398,7 +398,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by ciphertext with replicated keys, see DESIGN.md)",
+              "(the path would shard by ciphertext: every GPU holding the keys, B/G "
+              "ciphertexts each, results gathered afterwards)",
               file=sys.stderr)
         sys.exit(2)
 
