@@ -447,8 +447,8 @@ def main():
         F.check(L_.fhe_timer_start(ctx.h, None))
         for _ in range(args.steps):
             step()
-        ms = __import__("ctypes").c_float()
-        F.check(L_.fhe_timer_stop(ctx.h, None, __import__("ctypes").byref(ms)))
+        ms = ctypes.c_float()
+        F.check(L_.fhe_timer_stop(ctx.h, None, ctypes.byref(ms)))
         total_ms = float(ms.value)
         out = d_out.to_array(np.uint32, (B, N + 1))
         ph = F.tlwe_phase_u32(out, sk.s1).view(np.int32).astype(np.int64)
@@ -465,6 +465,15 @@ def main():
             "phase_error_std_log2": float(np.log2(np.std(err))),
             "phase_error_hist": {lab: int(c) for lab, c in zip(labels, hist)},
         }
+        if name == "mfma_split":
+            # matrix-core work actually issued (utilisation diagnostic, not the
+            # roofline figure): per forward transform 4 v_mfma_f32_32x32x16_f16
+            # (32768 flops each) and 24 v_mfma_f32_16x16x32_f16 (16384 each),
+            # six transforms per CMUX step
+            mf = p.n0 * 6 * (4 * 32768 + 24 * 16384)
+            rate = variants[name]["pbs_per_s"] * mf / 1e12
+            variants[name].update({"mfma_flops_per_pbs": mf, "mfma_tflops": rate,
+                                   "mfma_frac_of_fp16_dense_peak": rate / 2500.0})
         print(f"bench: {name}: timed {args.steps} steps, {total_ms / args.steps:.3f} ms/step, "
               f"{variants[name]['decrypt_failures']} decryption failures", file=sys.stderr,
               flush=True)
