@@ -40,6 +40,24 @@ int ensure_mfma(fhe_ctx *ctx, hipStream_t s)
 
 namespace fhe {
 
+// the key layouts launch_pbs1 builds on first use (split-MFMA key and
+// tables, or the two-wave key order), built now on stream s: afterwards
+// launch_pbs1 only reads context memory, so it may run on several streams
+int prepare_pbs1(fhe_ctx *ctx, hipStream_t s)
+{
+    if (!ctx->bk1) return FHE_OK;
+    if (ctx->variant == FHE_PBS_MFMA_SPLIT) return ensure_mfma(ctx, s);
+    if (!ctx->bk1w2_valid) {
+        const int n0 = int(ctx->p.n0);
+        if (!ctx->bk1w2)
+            HIP_TRY(hipMalloc(&ctx->bk1w2, bk1_device_elems(n0) * sizeof(cd)),
+                    "two-wave key allocation");
+        HIP_TRY(launch_bk1_to_w2(ctx->bk1, ctx->bk1w2, n0, s), "two-wave key layout");
+        ctx->bk1w2_valid = true;
+    }
+    return FHE_OK;
+}
+
 // lvl1 blind rotation (+ sample extraction) with the context's variant
 int launch_pbs1(fhe_ctx *ctx, const uint32_t *in, const uint32_t *tv, int tv_stride,
                 uint32_t *out, bool se, int B, hipStream_t s)
@@ -54,13 +72,8 @@ int launch_pbs1(fhe_ctx *ctx, const uint32_t *in, const uint32_t *tv, int tv_str
         return FHE_OK;
     }
     if (pbs1_waves(ctx, B) == 2) {
-        if (!ctx->bk1w2_valid) {
-            if (!ctx->bk1w2)
-                HIP_TRY(hipMalloc(&ctx->bk1w2, bk1_device_elems(n0) * sizeof(cd)),
-                        "two-wave key allocation");
-            HIP_TRY(launch_bk1_to_w2(ctx->bk1, ctx->bk1w2, n0, s), "two-wave key layout");
-            ctx->bk1w2_valid = true;
-        }
+        int rc = prepare_pbs1(ctx, s);
+        if (rc) return rc;
         HIP_TRY(launch_blind_rotate1w2(in, n0, ctx->bk1w2, tv, tv_stride, out, se, B, s),
                 "pbs lvl01 (two waves per ciphertext)");
         return FHE_OK;

@@ -153,6 +153,7 @@ inline int pbs1_waves(const fhe_ctx *ctx, int B)
 // context's variant and tuned shape (capi.hip)
 int launch_pbs1(fhe_ctx *ctx, const uint32_t *in, const uint32_t *tv, int tv_stride,
                 uint32_t *out, bool se, int B, hipStream_t s);
+int prepare_pbs1(fhe_ctx *ctx, hipStream_t s);
 
 inline int ks_target_n(const fhe_ctx *ctx, int kind) { return kind == 21 ? Lvl1::N : int(ctx->p.n0); }
 inline int ks_domain_words(int kind) { return kind == 10 ? Lvl1::N + 1 : Lvl2::N + 1; }

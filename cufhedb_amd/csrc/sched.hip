@@ -11,9 +11,15 @@
 // launches of up to max_rows rows: their inputs are gathered into device
 // staging buffers, the per-request test vector / threshold is replicated per
 // row (the kernels take one per ciphertext), one batched call runs, and the
-// results are scattered back.  Requests are served in submission order on
-// the context's stream; a request of more than max_rows rows runs alone,
-// straight on its own buffers.
+// results are scattered back.  A request of more than max_rows rows runs
+// alone, straight on its own buffers.  lvl1 launches go to the context's
+// stream and lvl2 comparisons to a second stream of the scheduler (default
+// priority), so a lvl1 batch and a lvl2 comparison share the CUs (each
+// kind in submission order; the second stream starts after the work
+// already on the context stream, and the context stream continues after
+// both).  The two kinds use disjoint staging and scratch: a comparison's
+// chain scratch is the context's, the lvl1 bootstrap only reads the keys,
+// whose lazily built layouts are built before the streams fork.
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -74,6 +80,8 @@ struct fhe_sched {
     uint32_t *in1 = nullptr, *tv1 = nullptr, *out1 = nullptr;
     uint64_t *a2 = nullptr, *b2 = nullptr;
     uint32_t *outc = nullptr;
+    hipStream_t s2 = nullptr;  // lvl2 comparisons
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 using namespace fhe;
@@ -164,17 +172,15 @@ int run_alone(fhe_sched *s, Req &r, hipStream_t st)
     return FHE_OK;
 }
 
-// serves every request not yet flushed, in order
-int flush_pending(fhe_sched *s)
+// serves every request not yet flushed, each kind in order on its stream
+int serve_pending(fhe_sched *s, hipStream_t st1, hipStream_t st2)
 {
-    const hipStream_t st = s->ctx->stream;
-    for (Req &r : s->reqs)
-        if (!r.flushed) HIP_TRY(hipEventRecord(r.start, st), "hipEventRecord");
+    auto stream_of = [&](int kind) { return kind == FHE_SCHED_PBS1 ? st1 : st2; };
     std::vector<Req *> group;
     int rows = 0;
     auto close_group = [&]() -> int {
         if (group.empty()) return FHE_OK;
-        const int rc = run_group(s, group, rows, st);
+        const int rc = run_group(s, group, rows, stream_of(group[0]->kind));
         group.clear();
         rows = 0;
         return rc;
@@ -184,7 +190,7 @@ int flush_pending(fhe_sched *s)
         r.flushed = true;
         if (r.B > s->max_rows) {
             SCHED_TRY(close_group());
-            SCHED_TRY(run_alone(s, r, st));
+            SCHED_TRY(run_alone(s, r, stream_of(r.kind)));
             continue;
         }
         if (!group.empty() && (!same_group(*group[0], r) || rows + r.B > s->max_rows))
@@ -193,6 +199,21 @@ int flush_pending(fhe_sched *s)
         rows += r.B;
     }
     return close_group();
+}
+
+int flush_pending(fhe_sched *s)
+{
+    const hipStream_t st1 = s->ctx->stream, st2 = s->s2;
+    for (Req &r : s->reqs)
+        if (!r.flushed) HIP_TRY(hipEventRecord(r.start, st1), "hipEventRecord");
+    SCHED_TRY(prepare_pbs1(s->ctx, st1));
+    HIP_TRY(hipEventRecord(s->fork, st1), "hipEventRecord");
+    HIP_TRY(hipStreamWaitEvent(st2, s->fork, 0), "hipStreamWaitEvent");
+    const int rc = serve_pending(s, st1, st2);
+    // the context stream continues after the comparisons, also after an error
+    HIP_TRY(hipEventRecord(s->join, st2), "hipEventRecord");
+    HIP_TRY(hipStreamWaitEvent(st1, s->join, 0), "hipStreamWaitEvent");
+    return rc;
 }
 
 int submit(fhe_sched *s, Req r, uint64_t *ticket)
@@ -223,6 +244,16 @@ int fhe_sched_create(fhe_ctx *ctx, int max_rows, fhe_sched **out)
     auto *s = new fhe_sched;
     s->ctx = ctx;
     s->max_rows = max_rows;
+    hipError_t e = hipStreamCreateWithFlags(&s->s2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->join, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        if (s->join) (void)hipEventDestroy(s->join);
+        if (s->fork) (void)hipEventDestroy(s->fork);
+        if (s->s2) (void)hipStreamDestroy(s->s2);
+        delete s;
+        return hip_fail(e, "fhe_sched_create");
+    }
     *out = s;
     return FHE_OK;
 }
@@ -232,6 +263,7 @@ int fhe_sched_destroy(fhe_sched *s)
     if (!s) return FHE_OK;
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
+    (void)hipStreamSynchronize(s->s2);
     for (Req &r : s->reqs) {
         if (r.start) (void)hipEventDestroy(r.start);
         if (r.done) (void)hipEventDestroy(r.done);
@@ -239,6 +271,9 @@ int fhe_sched_destroy(fhe_sched *s)
     void *bufs[] = {s->in1, s->tv1, s->out1, s->a2, s->b2, s->outc};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    (void)hipEventDestroy(s->fork);
+    (void)hipEventDestroy(s->join);
+    (void)hipStreamDestroy(s->s2);
     delete s;
     return FHE_OK;
 }
