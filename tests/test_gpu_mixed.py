@@ -70,8 +70,19 @@ def test_scheduler_matches_direct_calls(full672):
         else:
             tickets.append(sched.submit_cmp2(op, b.inp, b.aux, b.out, 20, F.LOGIC, b.B))
     sched.flush()
+    # the context stream continues after both kinds: a negation enqueued on
+    # it right after the flush (HomNOT, no bootstrap) reads the last
+    # comparison's finished result
+    last = batches[8]
+    snap = F.DeviceBuffer(last.B * (F.LVL1_N + 1) * 4)
+    F.check(F.lib().fhe_hom_not_dev(ctx.h, 1, last.out.ptr, snap.ptr, last.B, None))
     lat = [sched.wait(t) for t in tickets]
     assert all(ms > 0 for ms in lat)
+    ctx.sync()
+    neg = (np.uint64(1 << 32) - last.result().astype(np.uint64)) % np.uint64(1 << 32)
+    assert np.array_equal(snap.to_array(np.uint32, (last.B, F.LVL1_N + 1)),
+                          neg.astype(np.uint32))
+    snap.free()
     with pytest.raises(F.FheError):
         sched.wait(tickets[0])  # already waited for
     sched.close()
