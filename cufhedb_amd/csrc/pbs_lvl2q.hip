@@ -17,7 +17,8 @@
 //           a radix-4 pass over the registers, then the 64-point DFT over
 //           the lanes as three radix-4 passes: the first exchange by
 //           cross-lane register swaps (v_permlane32_swap /
-//           v_permlane16_swap), the other two by wave-local LDS transposes
+//           v_permlane16_swap) for batches up to kPermMaxBatch, through
+//           LDS above; the other two by wave-local LDS transposes
 //           (row strides 20 and 5 double2, conflict-free for gfx950's
 //           16-lanes-per-cycle b128 accesses).
 // Layouts: time domain wave w, lane L, register p holds z[L + 64 w + 256 p];
@@ -78,6 +79,7 @@ __device__ __forceinline__ cd cq(int p)
 // Forward: v[p] = z[lane + 64 wave + 256 p] in, Z[4 m' + wave] (layout above)
 // out.  xch: the workgroup's 4 x 256 double2 exchange area, which aliases the
 // four wave tiles (tiles = xch + wave * kTileQ).
+template <bool PERM = false>
 __device__ __forceinline__ void fft1024q_fwd(cd v[4], cd *xch, int lane, int wave, const TwQ &tw)
 {
     v[1] = cmul(v[1], cq(1));
@@ -104,12 +106,13 @@ __device__ __forceinline__ void fft1024q_fwd(cd v[4], cd *xch, int lane, int wav
     // step C: radix-4 over the registers (stride 64), twiddle W256^{L m1}
     dft4<1>(v[0], v[1], v[2], v[3]);
     twiddle4<false>(v, tw.t256);
-    lanes64_fwd<true>(v, xch + wave * kTileQ, lane, tw.t64, tw.t16);
+    lanes64_fwd<PERM>(v, xch + wave * kTileQ, lane, tw.t64, tw.t16);
 }
 
+template <bool PERM = false>
 __device__ __forceinline__ void fft1024q_inv(cd v[4], cd *xch, int lane, int wave, const TwQ &tw)
 {
-    lanes64_inv<true>(v, xch + wave * kTileQ, lane, tw.t64, tw.t16);
+    lanes64_inv<PERM>(v, xch + wave * kTileQ, lane, tw.t64, tw.t16);
     twiddle4<true>(v, tw.t256);
     dft4<-1>(v[0], v[1], v[2], v[3]);
     // all waves finished their tiles before the exchange overwrites them
@@ -185,6 +188,13 @@ __device__ __forceinline__ double digit2qp(uint64_t u, int d)
 
 constexpr int kQRow = 4 * 4 * 64;  // double2 per (i, row, column): [wave][q][lane]
 
+// The first lane transpose as register swaps (PERM) trades LDS traffic for
+// VALU work: measured A/B on one box, 75.3 against 77.3 ms per 4096
+// bootstraps, but 388.4 against 376.5 ms per 20480 and 2.28 against 2.22 s
+// for the 2^14-row digit-wise comparison.  Batches up to this size use it.
+constexpr int kPermMaxBatch = 8192;
+
+template <bool PERM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_blind_rotate2q(
     const uint32_t *__restrict__ in, int n0, const cd *__restrict__ bk,
     const cd *__restrict__ rot, const uint64_t *__restrict__ tv, int tv_stride, int tv_period,
@@ -236,7 +246,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 const int n = lane + 64 * wave + 256 * p;
                 v[p] = cd{digit2qp(prep2q(poly[n]), d), digit2qp(prep2q(poly[n + 1024]), d)};
             }
-            fft1024q_fwd(v, xch, lane, wave, tw);
+            fft1024q_fwd<PERM>(v, xch, lane, wave, tw);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 cmac(fd0[q], v[q], b0[q]);
@@ -254,14 +264,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             fd1[q] = cmul(fd1[q], g);
         }
         // each lane reads and updates only its own accumulator words
-        fft1024q_inv(fd0, xch, lane, wave, tw);
+        fft1024q_inv<PERM>(fd0, xch, lane, wave, tw);
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             const int n = lane + 64 * wave + 256 * p;
             acc[n] += trunc_torus64q(fd0[p].x);
             acc[n + 1024] += trunc_torus64q(fd0[p].y);
         }
-        fft1024q_inv(fd1, xch, lane, wave, tw);
+        fft1024q_inv<PERM>(fd1, xch, lane, wave, tw);
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             const int n = lane + 64 * wave + 256 * p;
@@ -349,8 +359,12 @@ hipError_t launch_blind_rotate2q(const uint32_t *in, int n0, const cd *bk, const
                                  bool sample_extract, int B, hipStream_t s, int tv_period)
 {
     if (B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_blind_rotate2q, dim3(B), dim3(256), 0, s, in, n0, bk, rot, tv, tv_stride,
-                       tv_period, out, int(sample_extract), B);
+    if (B <= kPermMaxBatch)
+        hipLaunchKernelGGL(k_blind_rotate2q<true>, dim3(B), dim3(256), 0, s, in, n0, bk, rot, tv,
+                           tv_stride, tv_period, out, int(sample_extract), B);
+    else
+        hipLaunchKernelGGL(k_blind_rotate2q<false>, dim3(B), dim3(256), 0, s, in, n0, bk, rot,
+                           tv, tv_stride, tv_period, out, int(sample_extract), B);
     return hipGetLastError();
 }
 

@@ -104,3 +104,22 @@ def test_bad_arguments(keys636, ctx636):
     for K in (0, 65):
         with pytest.raises(F.FheError):
             ctx636.pbs_many_lut(1, c0, tv, K)
+
+
+def test_full_batch_8192(keys636, ctx636):
+    """At the benchmark's batch (8192 ciphertexts, n0 = 636): two tables of
+    Z_8 messages per blind rotation, every output decrypting to its table
+    entry, and the one-table form equal to the plain PBS word for word."""
+    k = keys636
+    rng = np.random.default_rng(8192)
+    B = 8192
+    m = rng.integers(0, 8, B)
+    luts = rng.integers(0, 16, (2, 8))
+    tv = F.many_lut_testvector(luts << 28, 3)
+    c0 = k.enc_lvl0((m << 28).astype(np.uint32), index0=75000)
+    got = ctx636.pbs_many_lut(1, c0, tv, 2)
+    ph = F.tlwe_phase_u32(got.reshape(-1, F.LVL1_N + 1), k.sk.s1).reshape(B, 2)
+    dec = np.round(O.tdiff32(ph, 0) / (1 << 28)).astype(np.int64) % 16
+    assert np.array_equal(dec, luts.T[m])
+    one = ctx636.pbs_many_lut(1, c0, tv, 1)
+    assert np.array_equal(one[:, 0], ctx636.pbs_lvl01(c0, tv))
