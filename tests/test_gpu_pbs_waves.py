@@ -93,3 +93,22 @@ def test_autotune_waves_table(keys636, ctx636):
     with pytest.raises(F.FheError):
         ctx636.set_tune_table([(1, 3)], knob=KNOB)
     assert ctx636.tune_table(knob=KNOB) == []
+
+
+def test_two_wave_transpose_forms_agree(keys636, ctx636):
+    """Two-wave batches up to 256 run the lane transform's first transpose as
+    register swaps, larger ones through LDS; both move the same words, so a
+    256- and a 257-ciphertext batch agree word for word on their common
+    ciphertexts."""
+    k = keys636
+    rng = np.random.default_rng(257)
+    m = rng.integers(0, 8, 257)
+    tv = z8_testvector(rng.integers(0, 8, 8))
+    c0 = k.enc_lvl0((m << 28).astype(np.uint32), index0=77000)
+    ctx636.set_tune_table([(1, 2)], knob=KNOB)
+    try:
+        big = ctx636.pbs_lvl01(c0, tv)
+        small = ctx636.pbs_lvl01(c0[:256], tv)
+    finally:
+        ctx636.set_tune_table([], knob=KNOB)
+    assert np.array_equal(big[:256], small)
