@@ -157,3 +157,34 @@ def test_gate_rescale_matches_gate_then_rescale(full636):
                                             32, rows, None))
     for buf in enc + [out, mid]:
         buf.free()
+
+
+def test_groupby_full_table_2e14_rows(full636):
+    """The filter-aggregate query at its full size, 2^14 rows: every group
+    predicate and every aggregate equal to the plaintext query's."""
+    k, ctx = full636
+    rng = np.random.default_rng(8500)
+    rows, cutoff = 1 << 14, 2400
+    t = synthetic_table(rng, rows)
+    t["date"][:3] = [cutoff, cutoff + 1, 2556]
+    dd = F.DeviceBuffer.from_array(F.encrypt_digits(8501, 0, k.sk.s2, t["date"], 3, 4,
+                                                    k.params.alpha2))
+    c1 = F.DeviceBuffer.from_array(enc1(k, t["cat1"], 2, 8502))
+    c2 = F.DeviceBuffer.from_array(enc1(k, t["cat2"], 1, 8503))
+    consts = [F.DeviceBuffer.from_array(enc1(k, [v], b, 8504 + i))
+              for i, (v, b) in enumerate(((1, 2), (2, 2), (1, 1)))]
+    s = Q.groupby_filter(ctx, dd, c1, c2, *consts, rows, cutoff)
+    ctx.sync()
+    pred = Q.decrypt_predicates(s.out, rows, k.sk.s1)
+    ok = t["date"] <= cutoff
+    want = []
+    for g, (a, b) in enumerate(Q.GROUPS):
+        m = ok & (t["cat1"] == a) & (t["cat2"] == b)
+        assert np.array_equal(pred[g], m.astype(np.int64)), (a, b)
+        want.append((int(t["quantity"][m].sum()),
+                     int((t["price"][m] * (100 - t["discount"][m])).sum()), int(m.sum())))
+    got = Q.group_aggregates(pred, t["quantity"], t["price"], t["discount"])
+    assert np.array_equal(got, np.array(want, np.int64))
+    s.free()
+    for b in [dd, c1, c2] + consts:
+        b.free()
