@@ -134,10 +134,10 @@ def test_fastr_load(small, tmp_path):
     b.load_ksk(10, ksk10)
     rng = np.random.default_rng(3)
     bits = rng.integers(0, 2, (2, 48))
-    mu = np.uint32(F.LVL1_MU)
-    ca = F.tlwe_encrypt_u32(31, 0, sk.s1, np.where(bits[0] == 1, mu, -mu).astype(np.uint32),
+    mu, neg_mu = np.uint32(F.LVL1_MU), np.uint32((1 << 32) - F.LVL1_MU)
+    ca = F.tlwe_encrypt_u32(31, 0, sk.s1, np.where(bits[0] == 1, mu, neg_mu).astype(np.uint32),
                             p.alpha1)
-    cb = F.tlwe_encrypt_u32(32, 0, sk.s1, np.where(bits[1] == 1, mu, -mu).astype(np.uint32),
+    cb = F.tlwe_encrypt_u32(32, 0, sk.s1, np.where(bits[1] == 1, mu, neg_mu).astype(np.uint32),
                             p.alpha1)
     ga = a.gate(F.GATE_NAND, ca, cb)
     assert np.array_equal(ga, b.gate(F.GATE_NAND, ca, cb))
