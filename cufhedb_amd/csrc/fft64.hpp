@@ -335,9 +335,10 @@ __device__ __forceinline__ void transpose_reg_lane_hi_lds(cd v[4], cd *tile, int
 // lane-bits-4-5 transpose (PERM: lane swaps; otherwise through the tile),
 // the other two go through a wave-private LDS tile of kLaneTile double2
 // (row strides 20 and 5, conflict-free for 16-lane b128 accesses).  The
-// swaps move VALU work in for LDS traffic out: the four-wave lvl2 kernel,
-// which is the more LDS-bound, gains 2.5 %; the two-wave lvl1 kernel is
-// within +-4 % either way and keeps the tile.  t64 = W64^(lane & 15),
+// swaps move VALU work in for LDS traffic out, which pays on short launches
+// and costs on long ones: the kernels that use this transform instantiate
+// both forms and pick one by batch size (both move the same words, so the
+// results are identical).  t64 = W64^(lane & 15),
 // t16 = W16^(lane & 3).  Input: register r, lane L holds x_r[L]; output:
 // register q, lane l holds X_r[m] for r = l >> 4,
 // m = ((l >> 2) & 3) + 4 (l & 3) + 16 q.
