@@ -191,8 +191,10 @@ constexpr int kQRow = 4 * 4 * 64;  // double2 per (i, row, column): [wave][q][la
 // The first lane transpose as register swaps (PERM) trades LDS traffic for
 // VALU work: measured A/B on one box, 75.3 against 77.3 ms per 4096
 // bootstraps, but 388.4 against 376.5 ms per 20480 and 2.28 against 2.22 s
-// for the 2^14-row digit-wise comparison.  Batches up to this size use it.
-constexpr int kPermMaxBatch = 8192;
+// for the 2^14-row digit-wise comparison; on another, 148.0 against 146.9
+// ms per 8192 and 303.1 against 293.9 ms per 16384.  Batches up to this
+// size use it.
+constexpr int kPermMaxBatch = 4096;
 
 template <bool PERM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_blind_rotate2q(
