@@ -134,17 +134,21 @@ TUNE = {
                    (64, 256, 512, 1024, 2048)),
     "iks_split": ("workgroups per ciphertext of the lvl1 -> lvl0 key switch",
                   (16, 64, 256, 1024, 4096, 16384)),
+    "iks20_split": ("workgroups per ciphertext of the lvl2 -> lvl0 key switch",
+                    (16, 64, 256, 1024, 4096, 16384)),
 }
+KNOBS = (("iks21_tile", "TUNE_IKS21_TILE"), ("pbs1_waves", "TUNE_PBS1_WAVES"),
+         ("iks_split", "TUNE_IKS_SPLIT"), ("iks20_split", "TUNE_IKS20_SPLIT"))
 
 
 def autotune(F, ctx):
     """Shapes measured on this GPU per batch size and kept in the context for
     the query pipelines: the lvl2 -> lvl1 key switch's tile, the lvl1 blind
-    rotation's waves per ciphertext and the lvl1/lvl2 -> lvl0 key switches'
-    workgroups per ciphertext."""
+    rotation's waves per ciphertext and the lvl1 -> lvl0 and lvl2 -> lvl0
+    key switches' workgroups per ciphertext (each tuned on its own key)."""
     out = {}
-    for name, knob in (("iks21_tile", F.TUNE_IKS21_TILE), ("pbs1_waves", F.TUNE_PBS1_WAVES),
-                       ("iks_split", F.TUNE_IKS_SPLIT)):
+    for name, attr in KNOBS:
+        knob = getattr(F, attr)
         what, batches = TUNE[name]
         res = ctx.autotune(batches, knob=knob)
         print(f"bench: autotuned {what}: {ctx.tune_table(knob=knob)}", file=sys.stderr,
@@ -155,8 +159,8 @@ def autotune(F, ctx):
 
 
 def apply_tuning(F, ctx, tuned):
-    for name, knob in (("iks21_tile", F.TUNE_IKS21_TILE), ("pbs1_waves", F.TUNE_PBS1_WAVES),
-                       ("iks_split", F.TUNE_IKS_SPLIT)):
+    for name, attr in KNOBS:
+        knob = getattr(F, attr)
         ctx.set_tune_table(sorted((int(b), t) for b, t in tuned[name]["table"].items()),
                            knob=knob)
 
@@ -173,6 +177,29 @@ def timed_on(F, ctx, fn, reps):
     ms = ctypes.c_float()
     F.check(L_.fhe_timer_stop(ctx.h, None, ctypes.byref(ms)))
     return float(ms.value) / reps
+
+
+def pbs_lvl01_rate(F, ctx, sk, p, B, reps):
+    """FP64 lvl1 PBS/s at batch B on ctx's parameter set (the Z_8 workload
+    of the headline), for the like-for-like line beside the CPU baseline,
+    which runs the reference's own parameter set (n0 = 672)."""
+    rng = np.random.default_rng(23)
+    m = rng.integers(0, 8, B)
+    lut = rng.integers(0, 8, 8)
+    c0 = F.tlwe_encrypt_u32(31, 0, sk.s0, (m << 28).astype(np.uint32), p.alpha0)
+    d_in, d_tv = F.DeviceBuffer.from_array(c0), F.DeviceBuffer.from_array(z8_testvector(lut))
+    d_out = F.DeviceBuffer(B * (N + 1) * 4)
+    L_ = F.lib()
+    ms = timed_on(F, ctx, lambda: F.check(L_.fhe_pbs_lvl01_dev(ctx.h, d_in.ptr, d_tv.ptr, 0,
+                                                                d_out.ptr, B, None)), reps)
+    ph = F.tlwe_phase_u32(d_out.to_array(np.uint32, (B, N + 1)), sk.s1).view(np.int32)
+    fails = int(np.sum(np.round(ph.astype(np.int64) / (1 << 28)).astype(np.int64) % 16 != lut[m]))
+    for b in (d_in, d_tv, d_out):
+        b.free()
+    print(f"bench: fp64 lvl1 PBS at n0={p.n0}, batch {B}: {ms:.2f} ms, {B / ms * 1e3:.0f} PBS/s, "
+          f"{fails} decryption failures", file=sys.stderr, flush=True)
+    return {"workload": f"lvl1 PBS, batch {B}, n0={p.n0}, fp64 (same kernel as the headline)",
+            "ms_per_step": ms, "pbs_per_s": B / ms * 1e3, "decrypt_failures": fails}
 
 
 def run_compare_digits(F, ctx, sk, p, rows, reps):
@@ -438,7 +465,12 @@ def main():
     names = ["fp64", "mfma_split"] if args.variant == "both" else [args.variant]
     variants = {}
     for name in names:
-        ctx.set_pbs_variant(codes[name])
+        try:
+            ctx.set_pbs_variant(codes[name])
+        except F.FheError as e:
+            variants[name] = {"unsupported": str(e)}
+            print(f"bench: {name}: {e}", file=sys.stderr, flush=True)
+            continue
         for _ in range(args.warmup):
             step()
         ctx.sync()
@@ -478,12 +510,13 @@ def main():
               f"{variants[name]['decrypt_failures']} decryption failures", file=sys.stderr,
               flush=True)
     # the headline is the fastest variant that decrypts every output
-    exact = [k for k in variants if variants[k]["decrypt_failures"] == 0] or list(variants)
+    ran = [k for k in variants if "pbs_per_s" in variants[k]]
+    exact = [k for k in ran if variants[k]["decrypt_failures"] == 0] or ran
     best = max(exact, key=lambda k: variants[k]["pbs_per_s"])
     ms_per_step = variants[best]["ms_per_step"]
     value = variants[best]["pbs_per_s"]
     failures = variants[best]["decrypt_failures"]
-    if len(variants) == 2:
+    if len(ran) == 2:
         variants["mfma_split_over_fp64"] = (variants["mfma_split"]["pbs_per_s"] /
                                             variants["fp64"]["pbs_per_s"])
     ctx.set_pbs_variant(codes[best])
@@ -491,7 +524,10 @@ def main():
     fl = flops_per_pbs(p.n0)
     achieved = fl * B / (ms_per_step / 1e3) / 1e12
     roofline = {
+        # compute-bound: the FP64 VALU (MI355X's FP64 vector peak equals its
+        # FP64 matrix peak, 78.6 TFLOP/s; no matrix instruction is issued)
         "bound": "mfma",
+        "compute_unit": "fp64 valu" if best == "fp64" else "fp16 mfma + fp32/fp64 valu",
         "achieved": achieved,
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
@@ -539,8 +575,14 @@ def main():
     }
     ctx.close()
     res["queries"] = None
-    if not args.no_query or args.mixed:
+    # the headline kernel on the reference's parameter set (n0 = 672), for a
+    # like-for-like comparison with the CPU baseline (compiled TFHEpp, whose
+    # n0 is fixed at 672)
+    gpu672 = None
+    if not args.no_query or args.mixed or not args.no_cpu_baseline:
         octx, osk, op = operator_context(F)
+        gpu672 = pbs_lvl01_rate(F, octx, osk, op, B, 5)
+    if not args.no_query or args.mixed:
         res["autotune"] = autotune(F, octx)
         if not args.no_query:
             res["queries"] = run_queries(F, octx, osk, op, args.query_rows, args.query_reps)
@@ -551,6 +593,7 @@ def main():
             _, coal = mixed.run_coalesced(octx, osk, op, plan)
             res["mixed_stream"] = {"sequential": seq, "coalesced": coal}
             print(f"bench: mixed stream: {res['mixed_stream']}", file=sys.stderr, flush=True)
+    if gpu672 is not None:
         octx.close()
         if args.mixed:
             # the same plan at n0 = 636 with the comparisons digit-wise
@@ -575,6 +618,10 @@ def main():
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(threads)
+        if gpu672 is not None and res["cpu_baseline"]["kind"] == "reference":
+            # both at n0 = 672: GPU FP64 kernel vs compiled TFHEpp on the host
+            res["cpu_baseline"]["gpu_same_params"] = dict(
+                gpu672, gpu_over_cpu=gpu672["pbs_per_s"] / res["cpu_baseline"]["value"])
     else:
         res["cpu_baseline"] = None
     print(json.dumps(res))

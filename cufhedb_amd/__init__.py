@@ -44,6 +44,7 @@ PBS_MFMA_SPLIT = 1
 TUNE_IKS21_TILE = 0
 TUNE_PBS1_WAVES = 1
 TUNE_IKS_SPLIT = 2
+TUNE_IKS20_SPLIT = 3
 
 GATE_NAND, GATE_AND, GATE_OR, GATE_NOR, GATE_XNOR = range(5)
 LOGIC = 1
@@ -423,15 +424,15 @@ class Context:
 
     # ---- autotuning (fhe_autotune / fhe_tune_get / fhe_tune_set)
     TUNE_CANDIDATES = {TUNE_IKS21_TILE: (8, 16, 32), TUNE_PBS1_WAVES: (1, 2),
-                       TUNE_IKS_SPLIT: (1, 2, 4, 8)}
+                       TUNE_IKS_SPLIT: (1, 2, 4, 8), TUNE_IKS20_SPLIT: (1, 2, 4, 8)}
 
     def autotune(self, batches=(64, 256, 1024, 4096, 16384), knob=None):
         """Time every candidate of `knob` (TUNE_IKS21_TILE: ciphertexts per
         workgroup of the lvl2 -> lvl1 key switch, needs KSK 21;
         TUNE_PBS1_WAVES: waves per ciphertext of the lvl1 blind rotation,
-        needs BK lvl01; TUNE_IKS_SPLIT: workgroups per ciphertext of the
-        lvl1/lvl2 -> lvl0 key switches, needs KSK 10 or 20) at each batch
-        size and keep the fastest; returns
+        needs BK lvl01; TUNE_IKS_SPLIT / TUNE_IKS20_SPLIT: workgroups per
+        ciphertext of the lvl1 -> lvl0 / lvl2 -> lvl0 key switch, needs KSK
+        10 / 20) at each batch size and keep the fastest; returns
         {batch: (choice, {candidate: ms})}."""
         knob = TUNE_IKS21_TILE if knob is None else knob
         cand = self.TUNE_CANDIDATES[knob]

@@ -33,7 +33,8 @@ bool knob_of(int knob, Knob &k)
     switch (knob) {
     case FHE_TUNE_IKS21_TILE: k = {3, {8, 16, 32, 0}}; return true;
     case FHE_TUNE_PBS1_WAVES: k = {2, {1, 2, 0, 0}}; return true;
-    case FHE_TUNE_IKS_SPLIT: k = {4, {1, 2, 4, 8}}; return true;
+    case FHE_TUNE_IKS_SPLIT:
+    case FHE_TUNE_IKS20_SPLIT: k = {4, {1, 2, 4, 8}}; return true;
     default: return false;
     }
 }
@@ -110,16 +111,18 @@ int fhe_autotune(fhe_ctx *ctx, int knob, const int *batches, int n, int *choice,
         FHE_CHECK_ARG(batches[i] >= 1 && batches[i] <= (1 << 20) &&
                           (i == 0 || batches[i] > batches[i - 1]),
                       "fhe_autotune: batch sizes must be ascending, in [1, 2^20]");
-    // the kernel each knob shapes, and the key it needs: KSK 21; KSK 10 (or
-    // else 20) for the digit-position split; BK lvl01
-    const int iks_kind = knob == FHE_TUNE_IKS21_TILE ? 21
-                         : knob == FHE_TUNE_IKS_SPLIT ? (ctx->ksk[ksk_slot(10)] ? 10 : 20)
-                                                      : 0;
+    // the kernel each knob shapes, and the key it needs: KSK 21, 10 or 20;
+    // BK lvl01
+    const int iks_kind = knob == FHE_TUNE_IKS21_TILE   ? 21
+                         : knob == FHE_TUNE_IKS_SPLIT   ? 10
+                         : knob == FHE_TUNE_IKS20_SPLIT ? 20
+                                                        : 0;
     const bool iks = iks_kind != 0;
     if (iks ? !ctx->ksk[ksk_slot(iks_kind)] : !ctx->bk1) {
-        set_error("fhe_autotune: %s not loaded",
-                  knob == FHE_TUNE_IKS21_TILE ? "KSK 21"
-                  : knob == FHE_TUNE_IKS_SPLIT ? "KSK 10 or 20" : "BK lvl01");
+        if (iks)
+            set_error("fhe_autotune: KSK %d not loaded", iks_kind);
+        else
+            set_error("fhe_autotune: BK lvl01 not loaded");
         return FHE_ERR_NOT_LOADED;
     }
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");

@@ -22,6 +22,11 @@ namespace {
 // derived from the FP64 device key on first use after each lvl1 upload.
 int ensure_mfma(fhe_ctx *ctx, hipStream_t s)
 {
+    if ((!ctx->mfma_tab || !ctx->bk1m_valid) && g_capturing) {
+        set_error("split-MFMA key layout needed during graph capture: run one lvl1 bootstrap "
+                  "with this variant before capturing");
+        return FHE_ERR_ARG;
+    }
     if (!ctx->mfma_tab) {
         HIP_TRY(hipMalloc(&ctx->mfma_tab, mfma_tables_bytes()), "MFMA table allocation");
         HIP_TRY(launch_mfma_tables(ctx->mfma_tab, s), "MFMA tables");
@@ -48,6 +53,12 @@ int prepare_pbs1(fhe_ctx *ctx, hipStream_t s)
     if (!ctx->bk1) return FHE_OK;
     if (ctx->variant == FHE_PBS_MFMA_SPLIT) return ensure_mfma(ctx, s);
     if (!ctx->bk1w2_valid) {
+        // fhe_load_bk builds this layout with the key, so only a failed
+        // upload leaves it missing; never build it into a captured graph
+        if (g_capturing) {
+            set_error("two-wave key layout missing during graph capture");
+            return FHE_ERR_ARG;
+        }
         const int n0 = int(ctx->p.n0);
         if (!ctx->bk1w2)
             HIP_TRY(hipMalloc(&ctx->bk1w2, bk1_device_elems(n0) * sizeof(cd)),
@@ -169,6 +180,11 @@ int fhe_set_pbs_variant(fhe_ctx *ctx, int variant)
     FHE_CHECK_ARG(ctx, "fhe_set_pbs_variant: null context");
     FHE_CHECK_ARG(variant == FHE_PBS_FP64 || variant == FHE_PBS_MFMA_SPLIT,
                   "fhe_set_pbs_variant: unknown variant %d", variant);
+    if (variant == FHE_PBS_MFMA_SPLIT) {
+        set_error("fhe_set_pbs_variant: the split-MFMA variant is disabled: with an FP32 key "
+                  "its phase noise (std 2^28.3) exceeds the 2^27 decision margin");
+        return FHE_ERR_UNSUPPORTED;
+    }
     ctx->variant = variant;
     return FHE_OK;
 }
@@ -210,7 +226,17 @@ int fhe_load_bk(fhe_ctx *ctx, int level, const void *bk_host, int format)
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(tmp);
     HIP_TRY(e, "fhe_load_bk");
-    if (l1) ctx->bk1m_valid = ctx->bk1w2_valid = false;
+    if (l1) {
+        // the two-wave kernel's key order is built here, with the key, so
+        // that no later bootstrap (possibly inside a graph capture) has to
+        ctx->bk1m_valid = ctx->bk1w2_valid = false;
+        const int v = ctx->variant;
+        ctx->variant = FHE_PBS_FP64;
+        const int rc = prepare_pbs1(ctx, ctx->stream);
+        ctx->variant = v;
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "fhe_load_bk");
+    }
     return FHE_OK;
 }
 

@@ -184,6 +184,12 @@ int fhe_hom_msb_dev(fhe_ctx *ctx, int in_level, const void *in_dev, uint32_t *ou
     FHE_CHECK_ARG(ctx && in_dev && out_dev && B >= 0, "fhe_hom_msb_dev: bad argument");
     FHE_CHECK_ARG(in_level == 1 || in_level == 2, "fhe_hom_msb_dev: level 1 or 2");
     FHE_CHECK_ARG(plain_bits >= 1, "fhe_hom_msb_dev: plain_bits >= 1");
+    if (in_level == 2 && plain_bits > kMaxMsbBits2) {
+        set_error("fhe_hom_msb_dev: plain_bits %u above %d at level 2: the peeling steps shift "
+                  "the lvl2 bootstrap noise onto the decision margin (wrong results from 28 bits on)",
+                  plain_bits, kMaxMsbBits2);
+        return FHE_ERR_UNSUPPORTED;
+    }
     if (B == 0) return FHE_OK;
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     return hom_msb(ctx, in_level, in_dev, out_dev, plain_bits, result_type ? FHE_LOGIC : FHE_ARITHMETIC,
@@ -199,6 +205,12 @@ int fhe_compare_dev(fhe_ctx *ctx, int level, int op, const void *a_dev, const vo
     FHE_CHECK_ARG(plain_bits >= 1 && plain_bits + 1 <= (level == 1 ? 10u : 33u),
                   "fhe_compare_dev: plain_bits + 1 must be at most %d at level %d",
                   level == 1 ? 10 : 33, level);
+    if (level == 2 && plain_bits + 1 > kMaxMsbBits2) {
+        set_error("fhe_compare_dev: plain_bits %u above %d at level 2: the peeling steps shift "
+                  "the lvl2 bootstrap noise onto the decision margin (wrong results from 27 bits on)",
+                  plain_bits, kMaxMsbBits2 - 1);
+        return FHE_ERR_UNSUPPORTED;
+    }
     if (B == 0) return FHE_OK;
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     hipStream_t s = pick(ctx, stream);

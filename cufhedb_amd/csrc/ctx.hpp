@@ -55,7 +55,7 @@ struct fhe_ctx {
     size_t many_cap[2] = {};
     // tuned choices per knob (FHE_TUNE_*) and batch size (ascending batch
     // sizes; a batch uses the entry of the largest size <= it)
-    static constexpr int kTuneKnobs = 3;
+    static constexpr int kTuneKnobs = 4;
     static constexpr int kTuneMax = 16;
     int tune_batch[kTuneKnobs][kTuneMax] = {};
     int tune_val[kTuneKnobs][kTuneMax] = {};
@@ -140,7 +140,7 @@ inline int tuned(const fhe_ctx *ctx, int knob, int B, int dflt)
 inline int iks_tile(const fhe_ctx *ctx, int kind, int B)
 {
     if (kind == 21) return tuned(ctx, FHE_TUNE_IKS21_TILE, B, B < 512 ? 8 : B < 2048 ? 16 : 32);
-    return tuned(ctx, FHE_TUNE_IKS_SPLIT, B, 8);
+    return tuned(ctx, kind == 10 ? FHE_TUNE_IKS_SPLIT : FHE_TUNE_IKS20_SPLIT, B, 8);
 }
 
 // waves per ciphertext of the FP64 lvl1 blind rotation for a batch of B

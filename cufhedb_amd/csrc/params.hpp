@@ -14,6 +14,13 @@ namespace fhe {
 
 constexpr int kMaxN0 = 1024;  // kernels accept any n0 in [1, kMaxN0]
 
+// Widest lvl2 HomMSB input (plaintext bits including the comparison's sign
+// bit): each 5-bit peeling step shifts the previous identity bootstrap's
+// output noise (std 2^36.2 on the 64-bit torus) left by pb - 11, onto the
+// 2^57 decision margin of the 6-bit window; from 28 bits on outputs next to
+// a slot boundary flip (error rates from ~1e-4 at 28 bits to ~50 % at 33).
+constexpr int kMaxMsbBits2 = 27;
+
 struct Lvl1 {
     static constexpr int N = 1024;
     static constexpr int nbit = 10;

@@ -296,7 +296,13 @@ int fhe_sched_submit_cmp2(fhe_sched *s, int op, const uint64_t *a_dev, const uin
 {
     FHE_CHECK_ARG(op >= FHE_CMP_GT && op <= FHE_CMP_EQ, "fhe_sched_submit_cmp2: unknown comparison %d",
                   op);
-    FHE_CHECK_ARG(plain_bits >= 1 && plain_bits <= 32, "fhe_sched_submit_cmp2: plain_bits in [1, 32]");
+    FHE_CHECK_ARG(plain_bits >= 1, "fhe_sched_submit_cmp2: plain_bits >= 1");
+    if (plain_bits + 1 > uint32_t(kMaxMsbBits2)) {
+        set_error("fhe_sched_submit_cmp2: plain_bits %u above %d: lvl2 comparisons of wider "
+                  "values return wrong results (bootstrap noise reaches the decision margin)",
+                  plain_bits, kMaxMsbBits2 - 1);
+        return FHE_ERR_UNSUPPORTED;
+    }
     Req r{};
     r.kind = FHE_SCHED_CMP2;
     r.in = a_dev;

@@ -137,7 +137,10 @@ enum fhe_pbs_variant {
     FHE_PBS_MFMA_SPLIT = 1 /* forward digit transforms as radix-16 x 32 DFT matrix
                               products on fp16 MFMA (hi/lo limbs, FP32
                               accumulation), FP32 frequency-domain MAC, FP64
-                              inverse transforms; lvl1 PBS only */
+                              inverse transforms; lvl1 PBS only.  Currently
+                              refused (FHE_ERR_UNSUPPORTED): with the key in
+                              FP32 its phase noise (std 2^28.3) exceeds the
+                              Z_8 decision margin 2^27 (37 % wrong outputs) */
 };
 int fhe_set_pbs_variant(fhe_ctx *ctx, int variant);
 
@@ -266,8 +269,11 @@ int fhe_log_rescale_lvl1_dev(fhe_ctx *ctx, const uint32_t *in_dev, uint32_t *out
  * HEDB/comparison/extract_msb.cpp:241-259): encrypted top bit of a
  * plaintext of `plain_bits` bits held in the top bits of the torus, by 5-bit
  * peeling steps (shift, MSB gate, identity bootstrapping, subtract) down to
- * one MSB gate.  in_level 1: plain_bits <= 10; in_level 2: plain_bits <= 33,
- * through IKS lvl21 once at most 9 bits remain.  Output: lvl1 TLWEs.
+ * one MSB gate.  in_level 1: plain_bits <= 10; in_level 2: plain_bits <= 27,
+ * through IKS lvl21 once at most 9 bits remain (the reference's
+ * ImExtractMSB29/33 exist, but from 28 bits on the peeling steps shift the
+ * lvl2 bootstrap noise onto the decision margin and results are wrong at
+ * rates from ~1e-4 to ~50 %: FHE_ERR_UNSUPPORTED).  Output: lvl1 TLWEs.
  * Needs BK lvl01 + KSK 10, and for level 2 also BK lvl02 + KSK 20 and 21. */
 int fhe_hom_msb_dev(fhe_ctx *ctx, int in_level, const void *in_dev, uint32_t *out_dev,
                     uint32_t plain_bits, int result_type, int B, void *stream);
@@ -281,8 +287,9 @@ enum fhe_cmp {
 /* a op b for B pairs of TLWEs of `level` holding plain_bits-bit values at
  * scale 2^(w - plain_bits - 1) (one padding bit, as in
  * test/HE3_Query/HomComp.cu:75-80).  b_broadcast != 0 compares every a with
- * the single TLWE at b_dev (a shared or public threshold).  Output: lvl1
- * TLWEs, LOGIC or ARITHMETIC. */
+ * the single TLWE at b_dev (a shared or public threshold).  Level 1:
+ * plain_bits <= 9; level 2: plain_bits <= 26 (FHE_ERR_UNSUPPORTED above, see
+ * fhe_hom_msb_dev).  Output: lvl1 TLWEs, LOGIC or ARITHMETIC. */
 int fhe_compare_dev(fhe_ctx *ctx, int level, int op, const void *a_dev, const void *b_dev,
                     int b_broadcast, uint32_t *out_dev, uint32_t plain_bits, int result_type,
                     int B, void *stream);
@@ -332,10 +339,13 @@ enum fhe_tune_knob {
                                 rotation: 1, or 2 (small batches: each
                                 ciphertext's CMUX on two SIMDs); untuned: 2
                                 below 1024 ciphertexts */
-    FHE_TUNE_IKS_SPLIT = 2   /* workgroups per ciphertext of the lvl1/lvl2 ->
-                                lvl0 key switches (1, 2, 4, 8: the digit
+    FHE_TUNE_IKS_SPLIT = 2,  /* workgroups per ciphertext of the lvl1 -> lvl0
+                                key switch (kind 10; 1, 2, 4, 8: the digit
                                 positions split, sums added with integer
-                                atomics); untuned: 8 */
+                                atomics); untuned: 8; needs KSK 10 */
+    FHE_TUNE_IKS20_SPLIT = 3 /* the same for the lvl2 -> lvl0 key switch
+                                (kind 20: twice the digit positions, 64-bit
+                                inputs), tuned apart; untuned: 8; needs KSK 20 */
 };
 int fhe_autotune(fhe_ctx *ctx, int knob, const int *batches, int n, int *choice, float *ms);
 int fhe_tune_set(fhe_ctx *ctx, int knob, const int *batches, const int *tiles, int n);
@@ -377,7 +387,8 @@ int fhe_sched_destroy(fhe_sched *s);
 /* in_dev: B lvl0 TLWEs; tv_dev: one test vector (N words); out_dev: B lvl1 TLWEs */
 int fhe_sched_submit_pbs1(fhe_sched *s, const uint32_t *in_dev, const uint32_t *tv_dev,
                           uint32_t *out_dev, int B, uint64_t *ticket);
-/* a_dev: B lvl2 TLWEs; b_dev: one lvl2 TLWE; out_dev: B lvl1 TLWEs (a op b) */
+/* a_dev: B lvl2 TLWEs; b_dev: one lvl2 TLWE; out_dev: B lvl1 TLWEs (a op b);
+ * plain_bits <= 26 (FHE_ERR_UNSUPPORTED above, as fhe_compare_dev) */
 int fhe_sched_submit_cmp2(fhe_sched *s, int op, const uint64_t *a_dev, const uint64_t *b_dev,
                           uint32_t *out_dev, uint32_t plain_bits, int result_type, int B,
                           uint64_t *ticket);

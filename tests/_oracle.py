@@ -42,6 +42,11 @@ def orc():
     return _orc
 
 
+def ref_built():
+    """Whether the compiled reference exists (without loading it)."""
+    return os.path.exists(REF_SO)
+
+
 def ref():
     """The compiled reference, or None."""
     global _ref

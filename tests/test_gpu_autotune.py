@@ -79,29 +79,38 @@ def test_iks_split_bit_exact(full636, split):
     oracle directly."""
     k, ctx = full636
     rng = np.random.default_rng(100 + split)
-    knob = F.TUNE_IKS_SPLIT
+    knobs = (F.TUNE_IKS_SPLIT, F.TUNE_IKS20_SPLIT)
     x1 = rng.integers(0, 2 ** 32, (37, F.LVL1_N + 1), dtype=np.uint64).astype(np.uint32)
     x2 = rng.integers(0, 2 ** 63, (37, F.LVL2_N + 1), dtype=np.uint64) * np.uint64(2)
     try:
-        ctx.set_tune_table([(1, 1)], knob=knob)
+        for knob in knobs:
+            ctx.set_tune_table([(1, 1)], knob=knob)
         ref10, ref20 = ctx.iks(10, x1), ctx.iks(20, x2)
-        ctx.set_tune_table([(1, split)], knob=knob)
+        for knob in knobs:
+            ctx.set_tune_table([(1, split)], knob=knob)
         got10, got20 = ctx.iks(10, x1), ctx.iks(20, x2)
     finally:
-        ctx.set_tune_table([], knob=knob)
+        for knob in knobs:
+            ctx.set_tune_table([], knob=knob)
     assert np.array_equal(got10, ref10)
     assert np.array_equal(got20, ref20)
     for b in (0, 36):
         assert np.array_equal(got10[b], O.iks32(x1[b], k.ksk10, k.n0))
 
 
-def test_autotune_split_table(full636):
+@pytest.mark.parametrize("knob", [F.TUNE_IKS_SPLIT, F.TUNE_IKS20_SPLIT])
+def test_autotune_split_table(full636, knob):
+    # kinds 10 and 20 are tuned apart, each on its own key; tuning one
+    # leaves the other's table alone
     _, ctx = full636
-    knob = F.TUNE_IKS_SPLIT
+    other = F.TUNE_IKS20_SPLIT if knob == F.TUNE_IKS_SPLIT else F.TUNE_IKS_SPLIT
     try:
+        ctx.set_tune_table([(1, 2)], knob=other)
         res = ctx.autotune(batches=(16, 256, 4096), knob=knob)
-        print("\n  lvl1 -> lvl0 key-switch split:", res)
+        print(f"\n  key-switch split, knob {knob}:", res)
         assert [b for b, _ in ctx.tune_table(knob=knob)] == [16, 256, 4096]
         assert all(s in (1, 2, 4, 8) for _, s in ctx.tune_table(knob=knob))
+        assert ctx.tune_table(knob=other) == [(1, 2)]
     finally:
         ctx.set_tune_table([], knob=knob)
+        ctx.set_tune_table([], knob=other)

@@ -67,8 +67,9 @@ def test_compare_lvl1(full636, rng, bits):
 
 # Plain widths stay at or below 26 bits (27 with the sign): a peeling step
 # shifts the previous identity bootstrap's output noise left by pb - 6 - 5,
-# and the lvl2 PBS output noise (std ~2^35.7 on the 64-bit torus, set by
-# FP64 rounding of 64-bit products, as in the reference's FP64 transforms)
+# and the lvl2 PBS output noise (std ~2^36.2 on the 64-bit torus, set by
+# FP64 rounding of 64-bit products as in the reference's FP64 transforms,
+# times sqrt(2) from applying X^a - 1 in the frequency domain)
 # then reaches the 2^57 decision margin of the 6-bit window from about
 # pb = 28 on; windows next to a slot boundary (plaintexts 0 and -1) are the
 # first to flip.  The reference's ImExtractMSB29/33 share this limit.
@@ -128,3 +129,16 @@ def test_compare_rejects_bad_arguments(full636):
         F.compare(ctx, 1, 9, c, c, 4)              # unknown op
     with pytest.raises(F.FheError):
         F.hom_msb(ctx, 1, c, 11)
+
+
+def test_lvl2_width_limit_rejected(full672):
+    # level-2 comparisons of more than 26 bits (HomMSB inputs of more than
+    # 27) return wrong results at these parameters: refused before any launch
+    k, ctx = full672
+    c = np.zeros((2, 2049), np.uint64)
+    with pytest.raises(F.FheError) as e:
+        F.compare(ctx, 2, F.CMP_LT, c, c, 27)
+    assert e.value.code == F.FHE_ERR_UNSUPPORTED
+    with pytest.raises(F.FheError) as e:
+        F.hom_msb(ctx, 2, c, 28)
+    assert e.value.code == F.FHE_ERR_UNSUPPORTED

@@ -144,3 +144,51 @@ def test_capture_needing_scratch_fails_cleanly(keys636):
     a.free()
     o.free()
     ctx.close()
+
+
+def test_small_batch_capture_after_large_eager_run(keys636):
+    # A context that has only bootstrapped large batches (one wave per
+    # ciphertext) captures a small batch, which runs the two-wave kernel and
+    # its own key order.  That order is built when the key is loaded, so the
+    # capture records no layout work and an eager small batch before the
+    # first replay reads a complete key.
+    k = keys636
+    ctx = F.Context(k.params)
+    try:
+        ctx.load_bk(1, k.bk, F.BK_INT)
+        rng = np.random.default_rng(62)
+        lut = rng.integers(0, 8, 8)
+        j = np.arange(F.LVL1_N)
+        box = (j + 64) >> 7
+        tv = np.where(box == 8, -lut[box % 8], lut[box % 8]).astype(np.int64) << 28
+        d_tv = F.DeviceBuffer.from_array((tv % (1 << 32)).astype(np.uint32))
+        L = F.lib()
+
+        def batch(B, seed):
+            m = rng.integers(0, 8, B)
+            c0 = k.enc_lvl0((m << 28).astype(np.uint32), seed=seed)
+            return m, F.DeviceBuffer.from_array(c0), F.DeviceBuffer(B * W1 * 4)
+
+        mb, inb, outb = batch(4096, 63)
+        F.check(L.fhe_pbs_lvl01_dev(ctx.h, inb.ptr, d_tv.ptr, 0, outb.ptr, 4096, None))
+        ctx.sync()
+        ms, ins, outs = batch(64, 64)
+
+        def small():
+            F.check(L.fhe_pbs_lvl01_dev(ctx.h, ins.ptr, d_tv.ptr, 0, outs.ptr, 64, None))
+
+        graph = F.Graph(ctx, small)
+        eager_out = F.DeviceBuffer(64 * W1 * 4)
+        F.check(L.fhe_pbs_lvl01_dev(ctx.h, ins.ptr, d_tv.ptr, 0, eager_out.ptr, 64, None))
+        ctx.sync()
+        eager = eager_out.to_array(np.uint32, (64, W1))
+        for rep in range(3):
+            graph.launch()
+            ctx.sync()
+            assert np.array_equal(outs.to_array(np.uint32, (64, W1)), eager), rep
+        for m, o in ((ms, eager), (mb, outb.to_array(np.uint32, (4096, W1)))):
+            ph = F.tlwe_phase_u32(o, k.sk.s1).view(np.int32).astype(np.int64)
+            assert np.array_equal(np.round(ph / (1 << 28)).astype(np.int64) % 16, lut[m])
+        graph.destroy()
+    finally:
+        ctx.close()

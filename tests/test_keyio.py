@@ -22,7 +22,7 @@ from cufhedb_amd import keyio as K
 
 import _oracle as O
 
-ref_only = pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (no reference sources)")
+ref_only = pytest.mark.skipif(not O.ref_built(), reason="oracle/_ref not built (no reference sources)")
 
 
 @pytest.fixture(scope="module")
