@@ -35,8 +35,12 @@ $(OBJ)/%.cpp.o: $(SRC)/%.cpp $(HDRS)
 $(LIB): $(OBJS)
 	$(HIPCC) --offload-arch=$(ARCH) -shared -fPIC -o $@ $(OBJS) -lpthread
 
-# C++ drop-in usage example / test program (run by tests/test_gpu_dropin.py)
-dropin: $(OBJ)/dropin_demo
+# C++ drop-in usage examples / test programs (run by tests/test_gpu_dropin.py)
+dropin: $(OBJ)/dropin_demo $(OBJ)/dropin_refcalls
+
+$(OBJ)/dropin_refcalls: tests/cpp/dropin_refcalls.cpp include/cufhedb_amd/cuhedb.hpp include/cufhedb_amd.h $(LIB)
+	@mkdir -p $(OBJ)
+	$(CXX) $(CXXFLAGS) -Iinclude -o $@ $< -Lcufhedb_amd -lcufhedb_amd -pthread -Wl,-rpath,'$$ORIGIN/../cufhedb_amd'
 
 $(OBJ)/dropin_demo: tests/cpp/dropin_demo.cpp include/cufhedb_amd/cuhedb.hpp include/cufhedb_amd.h $(LIB)
 	@mkdir -p $(OBJ)

@@ -18,8 +18,10 @@ using namespace fhe;
 
 namespace {
 
-// The split-MFMA variant's key (FP32, its own frequency order) and tables,
-// derived from the FP64 device key on first use after each lvl1 upload.
+// The split-MFMA variant's key (three integer limbs per torus word, each
+// limb polynomial's transform in FP32, in its own frequency order) and
+// tables, derived from the FP64 device key when the variant is selected and
+// after each lvl1 upload while it is selected.
 int ensure_mfma(fhe_ctx *ctx, hipStream_t s)
 {
     if ((!ctx->mfma_tab || !ctx->bk1m_valid) && g_capturing) {
@@ -180,12 +182,15 @@ int fhe_set_pbs_variant(fhe_ctx *ctx, int variant)
     FHE_CHECK_ARG(ctx, "fhe_set_pbs_variant: null context");
     FHE_CHECK_ARG(variant == FHE_PBS_FP64 || variant == FHE_PBS_MFMA_SPLIT,
                   "fhe_set_pbs_variant: unknown variant %d", variant);
-    if (variant == FHE_PBS_MFMA_SPLIT) {
-        set_error("fhe_set_pbs_variant: the split-MFMA variant is disabled: with an FP32 key "
-                  "its phase noise (std 2^28.3) exceeds the 2^27 decision margin");
-        return FHE_ERR_UNSUPPORTED;
-    }
+    FHE_CHECK_ARG(!g_capturing, "fhe_set_pbs_variant: not during a graph capture");
     ctx->variant = variant;
+    // the variant's key layout is built now (when a key is loaded), so that
+    // no later bootstrap has to allocate or convert
+    if (variant == FHE_PBS_MFMA_SPLIT && ctx->bk1) {
+        HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+        if (int rc = ensure_mfma(ctx, ctx->stream)) return rc;
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "fhe_set_pbs_variant");
+    }
     return FHE_OK;
 }
 
@@ -232,8 +237,9 @@ int fhe_load_bk(fhe_ctx *ctx, int level, const void *bk_host, int format)
         ctx->bk1m_valid = ctx->bk1w2_valid = false;
         const int v = ctx->variant;
         ctx->variant = FHE_PBS_FP64;
-        const int rc = prepare_pbs1(ctx, ctx->stream);
+        int rc = prepare_pbs1(ctx, ctx->stream);
         ctx->variant = v;
+        if (rc == FHE_OK && v == FHE_PBS_MFMA_SPLIT) rc = ensure_mfma(ctx, ctx->stream);
         if (rc) return rc;
         HIP_TRY(hipStreamSynchronize(ctx->stream), "fhe_load_bk");
     }
@@ -250,6 +256,18 @@ int fhe_load_ksk(fhe_ctx *ctx, int kind, const uint32_t *ksk_host)
     if (int rc = alloc_ksk(ctx, kind)) return rc;
     HIP_TRY(hipMemcpy(ctx->ksk[slot], ksk_host, bytes, hipMemcpyHostToDevice), "KSK upload");
     return FHE_OK;
+}
+
+int fhe_bk_loaded(const fhe_ctx *ctx, int level)
+{
+    if (!ctx) return 0;
+    return (level == 1 ? ctx->bk1 : level == 2 ? ctx->bk2 : nullptr) != nullptr;
+}
+
+int fhe_ksk_loaded(const fhe_ctx *ctx, int kind)
+{
+    const int slot = ksk_slot(kind);
+    return ctx && slot >= 0 && ctx->ksk[slot] != nullptr;
 }
 
 int fhe_blind_rotate_lvl01_dev(fhe_ctx *ctx, const uint32_t *in_dev, const uint32_t *tv_dev,
@@ -434,9 +452,11 @@ struct TvSpec {
 };
 
 // in (lvl `in_level` TLWEs already offset, device) -> IKS to lvl0 -> PBS to
-// lvl `out_level` with the test vector -> body += post.
+// lvl `out_level` with the test vector -> body += post.  fixed: the lvl0
+// batch and test-vector buffers to use instead of the context's
+// grow-on-demand scratch (lvl1 output only).
 int run_bootstrap(fhe_ctx *ctx, int in_level, const void *in, int out_level, void *out, TvSpec tv,
-                  uint64_t post, int B, hipStream_t s)
+                  uint64_t post, int B, hipStream_t s, const GateScratch *fixed = nullptr)
 {
     const int kind = in_level == 1 ? 10 : 20;
     const int slot = ksk_slot(kind);
@@ -446,16 +466,18 @@ int run_bootstrap(fhe_ctx *ctx, int in_level, const void *in, int out_level, voi
         return FHE_ERR_NOT_LOADED;
     }
     if (B == 0) return FHE_OK;
-    int rc = grow(ctx->tmp0, ctx->tmp0_cap, size_t(B) * (ctx->p.n0 + 1));
+    int rc = FHE_OK;
+    if (!fixed) rc = grow(ctx->tmp0, ctx->tmp0_cap, size_t(B) * (ctx->p.n0 + 1));
     if (rc) return rc;
+    uint32_t *tmp0 = fixed ? fixed->tmp0 : ctx->tmp0;
+    uint32_t *tv1 = fixed ? fixed->tv1 : ctx->tv1;
     const int n0 = int(ctx->p.n0);
-    HIP_TRY(launch_iks(kind, in, ctx->ksk[slot], ctx->tmp0, n0, B, s, iks_tile(ctx, kind, B)),
-            "IKS");
+    HIP_TRY(launch_iks(kind, in, ctx->ksk[slot], tmp0, n0, B, s, iks_tile(ctx, kind, B)), "IKS");
     if (out_level == 1) {
-        HIP_TRY(tv.ide ? launch_tv_ide<uint32_t>(ctx->tv1, uint32_t(tv.value), Lvl1::N, s)
-                       : launch_tv_const<uint32_t>(ctx->tv1, uint32_t(tv.value), Lvl1::N, s),
+        HIP_TRY(tv.ide ? launch_tv_ide<uint32_t>(tv1, uint32_t(tv.value), Lvl1::N, s)
+                       : launch_tv_const<uint32_t>(tv1, uint32_t(tv.value), Lvl1::N, s),
                 "test vector");
-        rc = launch_pbs1(ctx, ctx->tmp0, ctx->tv1, 0, static_cast<uint32_t *>(out), true, B, s);
+        rc = launch_pbs1(ctx, tmp0, tv1, 0, static_cast<uint32_t *>(out), true, B, s);
         if (rc) return rc;
         if (post)
             HIP_TRY(launch_tlwe_add_body<uint32_t>(static_cast<uint32_t *>(out), Lvl1::N + 1,
@@ -465,7 +487,7 @@ int run_bootstrap(fhe_ctx *ctx, int in_level, const void *in, int out_level, voi
         HIP_TRY(tv.ide ? launch_tv_ide<uint64_t>(ctx->tv2, uint32_t(tv.value), Lvl2::N, s)
                        : launch_tv_const<uint64_t>(ctx->tv2, tv.value, Lvl2::N, s),
                 "test vector");
-        HIP_TRY(launch_blind_rotate2q(ctx->tmp0, n0, ctx->bk2, ctx->rot2, ctx->tv2, 0,
+        HIP_TRY(launch_blind_rotate2q(tmp0, n0, ctx->bk2, ctx->rot2, ctx->tv2, 0,
                                      static_cast<uint64_t *>(out), true, B, s),
                 "pbs lvl02");
         if (post)
@@ -502,46 +524,65 @@ int offset_input(fhe_ctx *ctx, int in_level, const void *in, uint64_t pre, int B
 // c = sa*a + sb*b + off -> IKS lvl10 -> PBS(tv) [-> body += post]
 int gate_pipeline(fhe_ctx *ctx, const uint32_t *a, const uint32_t *b, int sa, int sb, uint32_t off,
                   uint32_t u, bool ide, uint32_t scale_bits, uint32_t post, uint32_t *out, int B,
-                  hipStream_t s)
+                  hipStream_t s, const GateScratch *fixed = nullptr)
 {
     if (B == 0) return FHE_OK;
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
-    int rc = grow(ctx->tmp1, ctx->tmp1_cap, size_t(B) * (Lvl1::N + 1));
-    if (rc) return rc;
-    HIP_TRY(launch_tlwe1_lincomb(a, b ? b : a, sa, b ? sb : 0, off, ctx->tmp1, B, s), "lincomb");
-    return run_bootstrap(ctx, 1, ctx->tmp1, 1, out, TvSpec{ide, ide ? scale_bits : u}, post, B, s);
+    if (!fixed) {
+        int rc = grow(ctx->tmp1, ctx->tmp1_cap, size_t(B) * (Lvl1::N + 1));
+        if (rc) return rc;
+    }
+    uint32_t *tmp1 = fixed ? fixed->tmp1 : ctx->tmp1;
+    HIP_TRY(launch_tlwe1_lincomb(a, b ? b : a, sa, b ? sb : 0, off, tmp1, B, s), "lincomb");
+    return run_bootstrap(ctx, 1, tmp1, 1, out, TvSpec{ide, ide ? scale_bits : u}, post, B, s,
+                         fixed);
 }
 
 }  // namespace
 
-int fhe_gate_lvl1_dev(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint32_t *b_dev,
-                      uint32_t *out_dev, int result_type, int B, void *stream)
+}  // extern "C"
+
+namespace fhe {
+
+int gate_lvl1(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint32_t *b_dev,
+              uint32_t *out_dev, int result_type, int B, hipStream_t s, const GateScratch *fixed)
 {
-    FHE_CHECK_ARG(ctx && a_dev && b_dev && out_dev && B >= 0, "fhe_gate_lvl1_dev: bad argument");
     const uint32_t mu = Lvl1::mu;
     const bool logic = result_type != 0;
-    hipStream_t s = pick(ctx, stream);
     switch (gate) {
     case FHE_GATE_NAND:  // HomGate<-1,-1,+mu> + GateBootstrapping (TFHEpp gate.cpp:50-65)
-        return gate_pipeline(ctx, a_dev, b_dev, -1, -1, mu, mu, false, 0, 0, out_dev, B, s);
+        return gate_pipeline(ctx, a_dev, b_dev, -1, -1, mu, mu, false, 0, 0, out_dev, B, s, fixed);
     case FHE_GATE_NOR:
-        return gate_pipeline(ctx, a_dev, b_dev, -1, -1, 0u - mu, mu, false, 0, 0, out_dev, B, s);
+        return gate_pipeline(ctx, a_dev, b_dev, -1, -1, 0u - mu, mu, false, 0, 0, out_dev, B, s,
+                             fixed);
     case FHE_GATE_XNOR:
-        return gate_pipeline(ctx, a_dev, b_dev, -2, -2, 0u - 2 * mu, mu, false, 0, 0, out_dev, B, s);
+        return gate_pipeline(ctx, a_dev, b_dev, -2, -2, 0u - 2 * mu, mu, false, 0, 0, out_dev, B, s,
+                             fixed);
     case FHE_GATE_AND: {  // cuHEDB::HomAND: a + b - mu/2, tv = offset
         const uint32_t u = logic ? mu : mu << 1;
         return gate_pipeline(ctx, a_dev, b_dev, 1, 1, 0u - (mu >> 1), u, false, 0, logic ? 0 : u,
-                             out_dev, B, s);
+                             out_dev, B, s, fixed);
     }
     case FHE_GATE_OR: {  // cuHEDB::HomOR: a + b + mu/2
         const uint32_t u = logic ? mu : mu << 1;
         return gate_pipeline(ctx, a_dev, b_dev, 1, 1, mu >> 1, u, false, 0, logic ? 0 : u, out_dev,
-                             B, s);
+                             B, s, fixed);
     }
     default:
         set_error("fhe_gate_lvl1_dev: unknown gate %d", gate);
         return FHE_ERR_ARG;
     }
+}
+
+}  // namespace fhe
+
+extern "C" {
+
+int fhe_gate_lvl1_dev(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint32_t *b_dev,
+                      uint32_t *out_dev, int result_type, int B, void *stream)
+{
+    FHE_CHECK_ARG(ctx && a_dev && b_dev && out_dev && B >= 0, "fhe_gate_lvl1_dev: bad argument");
+    return gate_lvl1(ctx, gate, a_dev, b_dev, out_dev, result_type, B, pick(ctx, stream), nullptr);
 }
 
 int fhe_gate_rescale_lvl1_dev(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint32_t *b_dev,

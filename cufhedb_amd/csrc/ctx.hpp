@@ -21,9 +21,11 @@ struct fhe_ctx {
     fhe::cd *trgsw1 = nullptr;  // external-product TRGSW in device layout
     fhe::cd *trgsw2 = nullptr;
     int variant = FHE_PBS_FP64;
-    // split-MFMA variant: FP32 key in its frequency order (rebuilt from bk1
-    // on demand, bk1m_valid cleared by every lvl1 key upload) and its tables
-    float *bk1m = nullptr;
+    // split-MFMA variant: the key's three integer limbs, transformed, FP32,
+    // in its frequency order (built from bk1 when the variant is selected or
+    // a lvl1 key uploaded while it is; bk1m_valid cleared by every lvl1 key
+    // upload) and its tables
+    float *bk1m = nullptr;  // [i][row][col][limb][512 complex] FP32
     bool bk1m_valid = false;
     // two-waves-per-ciphertext kernel: bk1 permuted into its order (rebuilt
     // on demand, bk1w2_valid cleared by every lvl1 key upload)
@@ -154,6 +156,19 @@ inline int pbs1_waves(const fhe_ctx *ctx, int B)
 int launch_pbs1(fhe_ctx *ctx, const uint32_t *in, const uint32_t *tv, int tv_stride,
                 uint32_t *out, bool se, int B, hipStream_t s);
 int prepare_pbs1(fhe_ctx *ctx, hipStream_t s);
+
+// Buffers a lvl1 gate bootstrap works in: B lvl1 TLWEs (the gate's linear
+// combination), B lvl0 TLWEs (after the key switch) and one test vector.
+struct GateScratch {
+    uint32_t *tmp1;
+    uint32_t *tmp0;
+    uint32_t *tv1;
+};
+
+// HomNAND / NOR / XNOR / AND / OR of B pairs of lvl1 TLWEs (capi.hip): in the
+// context's grow-on-demand scratch (fixed == nullptr) or in `fixed`
+int gate_lvl1(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint32_t *b_dev,
+              uint32_t *out_dev, int result_type, int B, hipStream_t s, const GateScratch *fixed);
 
 inline int ks_target_n(const fhe_ctx *ctx, int kind) { return kind == 21 ? Lvl1::N : int(ctx->p.n0); }
 inline int ks_domain_words(int kind) { return kind == 10 ? Lvl1::N + 1 : Lvl2::N + 1; }

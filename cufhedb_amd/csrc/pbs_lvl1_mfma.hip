@@ -1,11 +1,17 @@
 // pbs_lvl1_mfma.hip -- lvl1 blind rotation with the forward (digit)
-// transforms on the fp16 matrix cores: the "split MFMA" variant.
+// transforms on the fp16 matrix cores and the bootstrapping key split into
+// integer limbs: the "split MFMA" variant.
 //
 // Same blind rotation, decomposition, rotation and sample extraction as
 // pbs_lvl1.hip (reference: thirdparty/TFHEpp/include/gatebootstrapping.hpp:15-43,
 // src/detwfa.cpp:21-30, src/trgsw.cpp:9-103; GPU counterpart
-// src/cuHEDB/fft_gpu/BlindRotate_gpu.cuh:401-531).  What changes is how the
-// six digit polynomials of every CMUX reach the frequency domain:
+// src/cuHEDB/fft_gpu/BlindRotate_gpu.cuh:401-531; the multiply-accumulate it
+// replaces: src/cuHEDB/fft_gpu/ExternalProduct_gpu.cuh:67-133,
+// thirdparty/TFHEpp/include/utils.hpp:83-108).  What changes is the
+// arithmetic of the external product.
+//
+// Forward transforms.  The six digit polynomials of every CMUX reach the
+// frequency domain as
 //
 //   Z[m2 + 16 m1] = sum_n1 W32^{n1 m1} T(n1, m2) sum_n2 A[m2][n2] z[n1 + 32 n2]
 //     A[m2][n2] = exp(i pi n2 (4 m2 + 1) / 32)      (radix-16 stage + twist)
@@ -20,12 +26,26 @@
 //            split hi + lo, products hi*hi + hi*lo + lo*hi: 24 MFMAs.
 // The data never leaves the wave: stage A consumes the digits in the lane
 // layout the rotation produces (lane L holds z[L + 64 j]); one LDS
-// transpose (16 x 34 float2, conflict-free) feeds stage B.  The key is held
-// in FP32 in stage B's output layout, the frequency-domain multiply-
-// accumulate is FP32, and the two inverse transforms stay FP64
-// (fft512_inv), fed through one conflict-free LDS permutation each.
-// Relative error of the forward transform: about 2^-23 (model and tests);
-// the PBS output phase error it adds is measured against the FP64 path.
+// transpose (16 x 34 float2, conflict-free) feeds stage B.  Relative RMS
+// error of the transform: 2^-22.7 (FP32 accumulation bounds it).
+//
+// Key limbs.  That error is far too large for a 32-bit key: the product
+// sums reach 2^40.7 (RMS) before the reduction mod 2^32, so a 2^-22.7
+// relative error leaves 2^18 per coefficient and 2^27 of output phase
+// noise, at the Z_8 decision margin.  The key's torus words are therefore
+// split into three signed integer limbs, K = K0 + 2^16 K1 + 2^24 K2 mod
+// 2^32 with K0 in [-2^15, 2^15), K1, K2 in [-2^7, 2^7), each limb's
+// polynomial transformed once (FP64, at key load) and stored in FP32.  Per
+// CMUX and output column, the frequency-domain products with each limb are
+// summed over the six rows in FP32 and brought back by one FP64 inverse
+// transform per limb; the limb products are exact integers:
+//   limbs 1, 2: |product| RMS 2^16.7, error RMS 2^-5.4 -> rounded exactly
+//               (0.5 is 21 standard deviations);
+//   limb 0:     RMS 2^24.7, error RMS 2^2.6, kept as is (weight 1);
+// and the three are recombined with the shifts and reduced mod 2^32 as
+// integers.  The external product thus carries ~2^2.6 of arithmetic error
+// per coefficient (the FP64 path: below 1), far under the key noise, so the
+// output phase noise equals the FP64 path's.
 #include "fft64.hpp"
 #include "launch.hpp"
 #include "params.hpp"
@@ -125,23 +145,76 @@ __global__ void k_mfma_tables(MfmaTables *tab)
     }
 }
 
-// FP64 device-layout key (lane s, register q <-> frequency fd_index1024(s, q),
-// already scaled by 1/512) -> FP32 in stage-B output order:
-// [i][row][col][k2 < 4][lane][2 complex], element k = 2 k2 + e of lane l is
-// frequency 4 (l >> 4) + (k & 3) + 16 (l & 15) + 256 (k >> 2).
-__global__ void k_bk1_to_mfma(const cd *__restrict__ src, float2 *__restrict__ dst, int npoly)
+// Limb split of a key torus word: K = K0 + 2^16 K1 + 2^24 K2 (mod 2^32),
+// K0 in [-2^15, 2^15), K1 and K2 in [-2^7, 2^7) (signed, balanced).
+constexpr int kLimbs = 3;
+__host__ __device__ constexpr int limb_shift(int b) { return b == 0 ? 0 : b == 1 ? 16 : 24; }
+__device__ __forceinline__ void split_limbs(int64_t k, double out[kLimbs])
 {
-    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gid >= size_t(npoly) * 512) return;
-    const size_t p = gid / 512;
-    const int rem = int(gid % 512);
-    const int k2 = rem >> 7, lane = (rem >> 1) & 63, e = rem & 1;
-    const int k = 2 * k2 + e;
-    const int m = 4 * (lane >> 4) + (k & 3) + 16 * (lane & 15) + 256 * (k >> 2);
-    const int q = m >> 6, r = m & 63;
-    const int s = (r & 7) * 8 + (r >> 3);  // fd_index1024(s, q) == m
-    const cd v = src[p * 512 + q * 64 + s];
-    dst[gid] = float2{float(v.x), float(v.y)};
+    const int64_t k0 = int64_t(int16_t(uint16_t(k)));
+    const int64_t r1 = (k - k0) >> 16;
+    const int64_t k1 = int64_t(int8_t(uint8_t(r1)));
+    const int64_t r2 = (r1 - k1) >> 8;
+    const int64_t k2 = int64_t(int8_t(uint8_t(r2)));
+    out[0] = double(k0);
+    out[1] = double(k1);
+    out[2] = double(k2);
+}
+
+// round to the nearest integer, mod 2^32, for |x| < 2^51: adding 1.5 * 2^52
+// rounds x to an integer (round-to-nearest-even) whose two's complement
+// residue mod 2^32 is the low mantissa word
+__device__ __forceinline__ uint32_t round_torus32(double x)
+{
+    return uint32_t(__double_as_longlong(x + 0x1.8p52));
+}
+
+// stage-B output position (float2 index within a 512-point polynomial,
+// [k2 < 4][lane][2]) of frequency m: lane l, element k of the forward
+// transform holds frequency 4 (l >> 4) + (k & 3) + 16 (l & 15) + 256 (k >> 2)
+__device__ __forceinline__ int stage_b_slot(int m)
+{
+    const int mm = m & 255;
+    const int l = (mm >> 4) + 16 * ((mm >> 2) & 3);
+    const int k = (mm & 3) + 4 * (m >> 8);
+    return (k >> 1) * 128 + l * 2 + (k & 1);
+}
+
+// FP64 device-layout key polynomial (lane s, register q <-> frequency
+// fd_index1024(s, q), scaled by 1/512) -> its torus words (exact: the FP64
+// inverse of an integer polynomial's transform rounds back to it) -> three
+// limb polynomials -> FP64 forward transforms, scaled by 1/512, stored in
+// FP32 in stage B's output order: [i][row][col][limb][k2 < 4][lane][2].
+// One wave per key polynomial.
+__global__ __launch_bounds__(64) void k_bk1_limbs_mfma(const cd *__restrict__ src,
+                                                        float2 *__restrict__ dst, int npoly)
+{
+    __shared__ cd tile[kFftLdsElems];
+    const int lane = threadIdx.x;
+    const int p = blockIdx.x;
+    if (p >= npoly) return;
+    const Twiddles1024 tw = make_twiddles1024(lane);
+    cd v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) v[q] = src[size_t(p) * 512 + q * 64 + lane];
+    fft512_inv(v, tile, lane, tw);
+    double lx[8][kLimbs], ly[8][kLimbs];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        split_limbs(int64_t(rint(v[j].x)), lx[j]);
+        split_limbs(int64_t(rint(v[j].y)), ly[j]);
+    }
+#pragma unroll 1
+    for (int b = 0; b < kLimbs; b++) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = cd{lx[j][b], ly[j][b]};
+        fft512_fwd(v, tile, lane, tw);
+        float2 *o = dst + (size_t(p) * kLimbs + b) * 512;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            o[stage_b_slot(fd_index1024(lane, q))] =
+                float2{float(v[q].x * (1.0 / 512)), float(v[q].y * (1.0 / 512))};
+    }
 }
 
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b)
@@ -266,7 +339,6 @@ k_blind_rotate1_mfma(const uint32_t *__restrict__ in, int n0, const float4 *__re
     const uint32_t *tvp = tv + size_t(ct) * tv_stride;
     float2 *buf = reinterpret_cast<float2 *>(tile);  // 576 x 16 B >= 544 / 512 float2 needed
 
-    const MfmaConsts mc = load_consts(tab, lane);
     const Twiddles1024 tw64 = make_twiddles1024(lane);
 
     const uint32_t bbar = 2u * N1 - (c0[n0] >> (32 - 1 - Lvl1::nbit));
@@ -280,15 +352,18 @@ k_blind_rotate1_mfma(const uint32_t *__restrict__ in, int n0, const float4 *__re
     }
     __syncthreads();
 
+    // key slice of one (row, column, limb): 256 float4
+    constexpr int kSlice = 256;
     constexpr uint32_t kRound = 1u << (32 - 2 - Lvl1::nbit);
     for (int i = 0; i < n0; i++) {
         const uint32_t abar = (c0[i] + kRound) >> (32 - 1 - Lvl1::nbit);
         if (abar == 0) continue;
-        const float4 *bki = bkm + size_t(i) * Lvl1::rows * 2 * 256;
-        float2 fd0[8], fd1[8];
+        const float4 *bki = bkm + size_t(i) * Lvl1::rows * 2 * kLimbs * kSlice;
+        // the six digit polynomials in the frequency domain (stage-B order:
+        // element k = 4 t + q of lane l is frequency 4 (l >> 4) + q + 16 (l & 15)
+        // + 256 t), kept in registers for the products with every key limb
+        float2 zf[Lvl1::rows][8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) fd0[k] = fd1[k] = float2{0.f, 0.f};
-#pragma unroll 1
         for (int r = 0; r < Lvl1::rows; r++) {
             const int c = r / Lvl1::l, d = r - c * Lvl1::l;
             const uint32_t *poly = acc + c * N1;
@@ -302,50 +377,61 @@ k_blind_rotate1_mfma(const uint32_t *__restrict__ in, int n0, const float4 *__re
                     _Float16(digit1m(rot_minus_one_m(poly, abar, n + 512), d));
             }
             f32x4 zr[2], zi[2];
+            // the stage constants are re-read (L1 / L2) per transform rather
+            // than held: registers go to the six transforms' outputs
+            const MfmaConsts mc = load_consts(tab, lane);
             mfma_fwd(b, mc, tab, buf, lane, zr, zi);
 #pragma unroll
-            for (int t = 0; t < 2; t++) {
-                // this tile's key entries (both output columns), FP32
-                float4 k0[2], k1[2];
+            for (int t = 0; t < 2; t++)
 #pragma unroll
-                for (int k2 = 0; k2 < 2; k2++) {
-                    k0[k2] = bki[(r * 2 + 0) * 256 + (2 * t + k2) * 64 + lane];
-                    k1[k2] = bki[(r * 2 + 1) * 256 + (2 * t + k2) * 64 + lane];
-                }
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int k = 4 * t + q;  // frequency 4 (lane>>4) + q + 16 (lane&15) + 256 t
-                    const float2 z{zr[t][q], zi[t][q]};
-                    const float4 p0 = k0[q >> 1], p1 = k1[q >> 1];
-                    const float2 key0 = (k & 1) ? float2{p0.z, p0.w} : float2{p0.x, p0.y};
-                    const float2 key1 = (k & 1) ? float2{p1.z, p1.w} : float2{p1.x, p1.y};
-                    fd0[k] = fmac2(fd0[k], z, key0);
-                    fd1[k] = fmac2(fd1[k], z, key1);
-                }
-            }
+                for (int q = 0; q < 4; q++) zf[r][4 * t + q] = float2{zr[t][q], zi[t][q]};
         }
-        // both columns: permute to the FP64 transform's layout, invert, accumulate
+        // per output column: for each key limb, the products summed over the
+        // rows (FP32), back to the time domain (FP64), rounded and shifted;
+        // the limbs' sum mod 2^32 goes into acc
 #pragma unroll 1
         for (int c = 0; c < 2; c++) {
+#pragma unroll 1
+            for (int lb = 0; lb < kLimbs; lb++) {
+                float2 fd[8];
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const int m = 4 * (lane >> 4) + (k & 3) + 16 * (lane & 15) + 256 * (k >> 2);
-                buf[perm_slot(m)] = c == 0 ? fd0[k] : fd1[k];
-            }
-            __syncthreads();
-            cd v[8];
+                for (int k = 0; k < 8; k++) fd[k] = float2{0.f, 0.f};
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const float2 f = buf[perm_slot(fd_index1024(lane, q))];
-                v[q] = cd{double(f.x), double(f.y)};
-            }
-            __syncthreads();
-            fft512_inv(v, tile, lane, tw64);
+                for (int r = 0; r < Lvl1::rows; r++) {
+                    const float4 *kr = bki + size_t((r * 2 + c) * kLimbs + lb) * kSlice + lane;
+                    float4 kv[4];
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int n = c * N1 + lane + 64 * j;
-                acc[n] += trunc_torus32(v[j].x);
-                acc[n + 512] += trunc_torus32(v[j].y);
+                    for (int k2 = 0; k2 < 4; k2++) kv[k2] = kr[k2 * 64];
+#pragma unroll
+                    for (int k = 0; k < 8; k++) {
+                        const float4 p = kv[k >> 1];
+                        const float2 key = (k & 1) ? float2{p.z, p.w} : float2{p.x, p.y};
+                        fd[k] = fmac2(fd[k], zf[r][k], key);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int m = 4 * (lane >> 4) + (k & 3) + 16 * (lane & 15) + 256 * (k >> 2);
+                    buf[perm_slot(m)] = fd[k];
+                }
+                __syncthreads();
+                cd v[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const float2 f = buf[perm_slot(fd_index1024(lane, q))];
+                    v[q] = cd{double(f.x), double(f.y)};
+                }
+                __syncthreads();
+                fft512_inv(v, tile, lane, tw64);
+                // the digits of every row are already in zf, so acc may
+                // take each limb's share at once (lane-private words)
+                const int sh = limb_shift(lb);
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int n = c * N1 + lane + 64 * j;
+                    acc[n] += round_torus32(v[j].x) << sh;
+                    acc[n + 512] += round_torus32(v[j].y) << sh;
+                }
             }
         }
         __syncthreads();
@@ -410,7 +496,7 @@ hipError_t launch_fwd_digits_mfma(const int32_t *poly, const void *tab, double *
 }
 
 size_t mfma_tables_bytes() { return sizeof(MfmaTables); }
-size_t bk1_mfma_floats(int n0) { return size_t(n0) * Lvl1::rows * 2 * 512 * 2; }
+size_t bk1_mfma_floats(int n0) { return size_t(n0) * Lvl1::rows * 2 * kLimbs * 512 * 2; }
 
 hipError_t launch_mfma_tables(void *tab, hipStream_t s)
 {
@@ -421,8 +507,7 @@ hipError_t launch_mfma_tables(void *tab, hipStream_t s)
 hipError_t launch_bk1_to_mfma(const cd *bk_dev, float *bkm, int n0, hipStream_t s)
 {
     const int npoly = n0 * Lvl1::rows * 2;
-    const size_t n = size_t(npoly) * 512;
-    hipLaunchKernelGGL(k_bk1_to_mfma, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, bk_dev,
+    hipLaunchKernelGGL(k_bk1_limbs_mfma, dim3(npoly), dim3(64), 0, s, bk_dev,
                        reinterpret_cast<float2 *>(bkm), npoly);
     return hipGetLastError();
 }

@@ -121,6 +121,12 @@ int fhe_load_bk(fhe_ctx *ctx, int level /* 1 or 2 */, const void *bk_host, int f
 /* Key-switching key upload (host TFHEpp layout, kind 10/20/21).  The
  * reference keeps these on the host and switches on the CPU. */
 int fhe_load_ksk(fhe_ctx *ctx, int kind, const uint32_t *ksk_host);
+/* 1 if the context holds a bootstrapping key of `level` (1, 2) / a
+ * key-switching key of `kind` (10, 20, 21), else 0.  No reference
+ * counterpart (the reference reads its keys from the caller's EvalKey on
+ * every call); the C++ drop-in uses them to upload an EvalKey's keys once. */
+int fhe_bk_loaded(const fhe_ctx *ctx, int level);
+int fhe_ksk_loaded(const fhe_ctx *ctx, int kind);
 /* Key-switching key generated on the device (ikskgen,
  * thirdparty/TFHEpp/src/cloudkey.cpp:54-70), bit-identical to fhe_keygen_ksk
  * for the same seed and keys: the device draws the masks and <a, s>, the
@@ -136,12 +142,16 @@ enum fhe_pbs_variant {
     FHE_PBS_FP64 = 0,      /* FP64 negacyclic FFT, LDS-resident (plain method) */
     FHE_PBS_MFMA_SPLIT = 1 /* forward digit transforms as radix-16 x 32 DFT matrix
                               products on fp16 MFMA (hi/lo limbs, FP32
-                              accumulation), FP32 frequency-domain MAC, FP64
-                              inverse transforms; lvl1 PBS only.  Currently
-                              refused (FHE_ERR_UNSUPPORTED): with the key in
-                              FP32 its phase noise (std 2^28.3) exceeds the
-                              Z_8 decision margin 2^27 (37 % wrong outputs) */
+                              accumulation); key split into three integer
+                              limbs (16 + 8 + 8 bits), FP32 frequency-domain
+                              MAC per limb, one FP64 inverse transform per
+                              limb and column, limb products rounded to
+                              integers and recombined mod 2^32; lvl1 PBS only */
 };
+/* Selects the lvl1 blind-rotation variant for every later lvl1 bootstrap of
+ * the context (plain PBS, gates, comparisons, scheduler).  The variant's key
+ * layout is built here when a key is loaded (and by fhe_load_bk afterwards),
+ * so bootstraps never allocate; not allowed during a graph capture. */
 int fhe_set_pbs_variant(fhe_ctx *ctx, int variant);
 
 /* ---- batched hot path on device buffers -------------------------------- */

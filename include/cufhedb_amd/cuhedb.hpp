@@ -39,10 +39,14 @@
 #pragma once
 
 #include <array>
+#include <chrono>
 #include <cstdint>
+#include <initializer_list>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../cufhedb_amd.h"
@@ -91,6 +95,13 @@ class DeviceBuffer {
     U *as() const { return static_cast<U *>(p_); }
     void upload(const void *src, size_t bytes) { check(fhe_memcpy_h2d(p_, src, bytes), "H2D"); }
     void download(void *dst, size_t bytes) const { check(fhe_memcpy_d2h(dst, p_, bytes), "D2H"); }
+    // forget the allocation without freeing it (objects alive until the
+    // process ends, whose destructors may run after the HIP runtime's)
+    void leak()
+    {
+        p_ = nullptr;
+        cap_ = 0;
+    }
 
   private:
     void *p_ = nullptr;
@@ -120,11 +131,49 @@ class Runtime {
 
     void load_bk(int level, const void *bk, int format) { check(fhe_load_bk(ctx_, level, bk, format), "fhe_load_bk"); }
     void load_ksk(int kind, const uint32_t *ksk) { check(fhe_load_ksk(ctx_, kind, ksk), "fhe_load_ksk"); }
+    bool has_bk(int level) const { return fhe_bk_loaded(ctx_, level) == 1; }
+    bool has_ksk(int kind) const { return fhe_ksk_loaded(ctx_, kind) == 1; }
+    // held by the per-ciphertext (reference-signature) calls, which the
+    // reference's callers make from several OpenMP threads at once (one
+    // stream_id each): they serialise on the context
+    std::mutex &mutex() const { return mu_; }
 
   private:
     fhe_params params_{};
     fhe_ctx *ctx_ = nullptr;
+    mutable std::mutex mu_;
 };
+
+namespace detail {
+inline Runtime *&default_slot()
+{
+    static Runtime *rt = nullptr;
+    return rt;
+}
+inline std::mutex &default_mu()
+{
+    static std::mutex m;
+    return m;
+}
+}  // namespace detail
+
+// The Runtime behind the reference-signature calls (the reference's global
+// singletons cufftlvl1 / cufftlvl2, gatebootstrap_gpu.cuh:24-25, which live
+// for the whole program): the one given to set_default_runtime, or else one
+// created on first use on device 0 with the reference's parameter set
+// (n0 = 672, as its fixed TLWE<lvl0param> of 673 words) and kept until the
+// process ends.
+inline void set_default_runtime(Runtime &rt)
+{
+    std::lock_guard<std::mutex> lk(detail::default_mu());
+    detail::default_slot() = &rt;
+}
+inline Runtime &default_runtime()
+{
+    std::lock_guard<std::mutex> lk(detail::default_mu());
+    if (!detail::default_slot()) detail::default_slot() = new Runtime(FHE_PRESET_REF672, 0);
+    return *detail::default_slot();
+}
 
 // Coalescing scheduler over a Runtime (fhe_sched_*): queue device-buffer
 // requests, flush() serves them in merged full-GPU launches, wait() blocks
@@ -197,30 +246,59 @@ class Graph {
 
 namespace cufft {
 
-// Same member names as the reference class; P::n selects lvl1 or lvl2.
+// Same member names, argument order and meaning as the reference class;
+// P::n selects lvl1 or lvl2.
 template <class P>
 class CuFFT_Torus {
     static_assert(P::n == 1024 || P::n == 2048, "lvl1 (N=1024) or lvl2 (N=2048)");
     using T = typename P::T;
     static constexpr int kLevel = P::n == 1024 ? 1 : 2;
+    using TRLWE_ = std::array<std::array<T, P::n>, 2>;
+    using TRGSWFFT_ = std::array<std::array<std::array<double, P::n>, 2>, (P::k + 1) * P::l>;
 
   public:
     const int32_t _2N = 2 * P::n, N = P::n, Ns2 = P::n / 2;
 
-    // num_stream kept for source compatibility (one context serves all ids)
+    // CuFFT_Torus(const int num_stream = 20) (cufft_gpu.cuh:135): on the
+    // process-wide default Runtime (created on first use, not here, so that
+    // global instances like the reference's cufftlvl1 / cufftlvl2 touch no
+    // GPU during static initialisation).  num_stream is kept for source
+    // compatibility: one context serves every stream_id.
+    explicit CuFFT_Torus(const int num_stream = 20) : num_stream_(num_stream) {}
+    // on an explicit Runtime
     explicit CuFFT_Torus(cufhedb_amd::Runtime &rt, const int num_stream = 20)
-        : rt_(rt), num_stream_(num_stream) {}
+        : rt_(&rt), num_stream_(num_stream) {}
+    ~CuFFT_Torus()
+    {
+        if (!rt_) return;
+        if (on_default_) {
+            din_.leak();
+            dtv_.leak();
+            dout_.leak();
+        }
+    }
+    CuFFT_Torus(const CuFFT_Torus &) = delete;
+    CuFFT_Torus &operator=(const CuFFT_Torus &) = delete;
+
+    cufhedb_amd::Runtime &runtime()
+    {
+        if (!rt_) {
+            rt_ = &cufhedb_amd::default_runtime();
+            on_default_ = true;
+        }
+        return *rt_;
+    }
 
     // LoadBK (cufft_gpu.cuh:175-184): TFHEpp BootstrappingKeyFFT, any n0
     template <size_t NK>
     void LoadBK(const std::array<std::array<std::array<std::array<double, P::n>, 2>,
                                             (P::k + 1) * P::l>, NK> &bkfft)
     {
-        if (NK != rt_.n0()) throw cufhedb_amd::Error(FHE_ERR_ARG, "LoadBK: key dimension != n0");
-        rt_.load_bk(kLevel, bkfft.data(), FHE_BK_FFT);
+        if (NK != runtime().n0()) throw cufhedb_amd::Error(FHE_ERR_ARG, "LoadBK: key dimension != n0");
+        runtime().load_bk(kLevel, bkfft.data(), FHE_BK_FFT);
     }
-    void LoadBK(const double *bkfft) { rt_.load_bk(kLevel, bkfft, FHE_BK_FFT); }
-    void LoadBKInt(const T *bk) { rt_.load_bk(kLevel, bk, FHE_BK_INT); }
+    void LoadBK(const double *bkfft) { runtime().load_bk(kLevel, bkfft, FHE_BK_FFT); }
+    void LoadBKInt(const T *bk) { runtime().load_bk(kLevel, bk, FHE_BK_INT); }
 
     // GateBootstrappingTLWE2TLWEFFT_st (cufft_gpu.cuh:250-271): constant test
     // vector u, lvl0 input of n0+1 words.
@@ -229,7 +307,8 @@ class CuFFT_Torus {
                                           const uint32_t /*stream_id*/)
     {
         std::vector<T> tv(P::n, u);
-        pbs_batch(tlwe.data(), in.data(), tv.data(), 0, 1);
+        std::lock_guard<std::mutex> lk(runtime().mutex());
+        pbs_batch_locked(tlwe.data(), in.data(), tv.data(), 0, 1);
     }
     // IdeGateBootstrappingTLWE2TLWEFFT_st (cufft_gpu.cuh:273-294): test
     // vector 2^scale_bits * (i >> 6)  (BlindRotate_gpu.cuh:124-161)
@@ -239,25 +318,112 @@ class CuFFT_Torus {
     {
         std::vector<T> tv(P::n);
         for (uint32_t i = 0; i < P::n; i++) tv[i] = (T(1) << scale_bits) * T(i >> 6);
-        pbs_batch(tlwe.data(), in.data(), tv.data(), 0, 1);
+        std::lock_guard<std::mutex> lk(runtime().mutex());
+        pbs_batch_locked(tlwe.data(), in.data(), tv.data(), 0, 1);
     }
     // ExternalProduct_st (cufft_gpu.cuh:228-248)
-    void ExternalProduct_st(std::array<std::array<T, P::n>, 2> &res,
-                            const std::array<std::array<T, P::n>, 2> &trlwe,
-                            const std::array<std::array<std::array<double, P::n>, 2>,
-                                             (P::k + 1) * P::l> &trgswfft,
+    void ExternalProduct_st(TRLWE_ &res, const TRLWE_ &trlwe, const TRGSWFFT_ &trgswfft,
                             const uint32_t /*stream_id*/)
     {
-        external_product_batch(res[0].data(), trlwe[0].data(), &trgswfft[0][0][0], 1);
+        std::lock_guard<std::mutex> lk(runtime().mutex());
+        last_trlwe_ = trlwe;
+        last_trgsw_ = trgswfft;
+        has_last_ = true;
+        external_product_locked(res[0].data(), trlwe[0].data(), &trgswfft[0][0][0], 1);
     }
-    // ifft_st / fft_st (cufft_gpu.cuh:187-212): TwistIFFT / TwistFFT
+    // MoveBuf (cufft_gpu.cuh:165-173): the per-row frequency-domain products
+    // of the last ExternalProduct_st call, FD(decomposition row r of the
+    // TRLWE) * TRGSW row r, column c, into trgswfft[r][c], in TFHEpp's FD
+    // layout ([re | im], spqlios order; the reference copies its kernel's
+    // internal buffer, in that kernel's own order).  Zeros before any call.
+    void MoveBuf(TRGSWFFT_ &trgswfft, const int /*stream_id*/ = 0)
+    {
+        std::lock_guard<std::mutex> lk(runtime().mutex());
+        if (!has_last_) {
+            for (auto &row : trgswfft)
+                for (auto &col : row) col.fill(0.0);
+            return;
+        }
+        std::array<T, P::n> dig;
+        std::array<double, P::n> fd;
+        for (int c = 0; c < 2; c++)
+            for (int d = 0; d < int(P::l); d++) {
+                const int r = c * int(P::l) + d;
+                for (uint32_t i = 0; i < P::n; i++) dig[i] = digit(last_trlwe_[c][i], d);
+                transform_locked(fd.data(), dig.data(), 1, true);
+                for (int oc = 0; oc < 2; oc++)
+                    mul_in_fd(trgswfft[r][oc].data(), fd.data(), last_trgsw_[r][oc].data());
+            }
+    }
+    // ifft_st / fft_st (cufft_gpu.cuh:187-212): TwistIFFT / TwistFFT.  As in
+    // the reference, each also sets the multiplier of PolyMul_st: the
+    // polynomial ifft_st transformed, or the frequency-domain vector fft_st
+    // was given.
     void ifft_st(std::array<double, P::n> &res, const std::array<T, P::n> &a, const uint32_t)
     {
-        transform(res.data(), a.data(), 1, true);
+        std::lock_guard<std::mutex> lk(runtime().mutex());
+        transform_locked(res.data(), a.data(), 1, true);
+        mult_fd_.assign(res.begin(), res.end());
     }
     void fft_st(std::array<T, P::n> &res, const std::array<double, P::n> &a, const uint32_t)
     {
-        transform(res.data(), a.data(), 1, false);
+        std::lock_guard<std::mutex> lk(runtime().mutex());
+        transform_locked(res.data(), a.data(), 1, false);
+        mult_fd_.assign(a.begin(), a.end());
+    }
+    // PolyMul_st (cufft_gpu.cuh:214-226; kernel ExternalProduct_gpu.cuh:136-170):
+    // each polynomial of trlwe times the current multiplier (initially the
+    // polynomial with every coefficient 1, as the reference's constructor
+    // sets it), TwistIFFT -> MulInFD -> TwistFFT with truncation
+    void PolyMul_st(TRLWE_ &res, const TRLWE_ &trlwe, const uint32_t /*stream_id*/)
+    {
+        std::lock_guard<std::mutex> lk(runtime().mutex());
+        if (mult_fd_.empty()) {
+            std::array<T, P::n> ones;
+            ones.fill(T(1));
+            mult_fd_.resize(P::n);
+            transform_locked(mult_fd_.data(), ones.data(), 1, true);
+        }
+        std::array<double, P::n> fa, prod;
+        for (int c = 0; c < 2; c++) {
+            transform_locked(fa.data(), trlwe[c].data(), 1, true);
+            mul_in_fd(prod.data(), fa.data(), mult_fd_.data());
+            transform_locked(res[c].data(), prod.data(), 1, false);
+        }
+    }
+
+    // ---- timing entry points (cufft_gpu.cuh:297-433): run the operation
+    // test_num times, cost = total wall time in microseconds.  This build has
+    // one (fused) kernel per operation, so the *_withoutFusion_test forms time
+    // the same path.
+    void ifft_test(std::array<double, P::n> &res, const std::array<T, P::n> &a, uint32_t test_num,
+                   double &cost)
+    {
+        cost = timed(test_num, [&] { ifft_st(res, a, 0); });
+    }
+    template <class In>
+    void GateBootstrappingTLWE2TLWEFFT_test(std::array<T, P::n + 1> &tlwe, const In &in,
+                                            const T u, uint32_t test_num, double &cost)
+    {
+        cost = timed(test_num, [&] { GateBootstrappingTLWE2TLWEFFT_st(tlwe, in, u, 0); });
+    }
+    template <class In>
+    void GateBootstrappingTLWE2TLWEFFT_withoutFusion_test(std::array<T, P::n + 1> &tlwe,
+                                                          const In &in, const T u,
+                                                          uint32_t test_num, double &cost)
+    {
+        GateBootstrappingTLWE2TLWEFFT_test(tlwe, in, u, test_num, cost);
+    }
+    void ExternalProduct_test(TRLWE_ &res, const TRLWE_ &trlwe, const TRGSWFFT_ &trgswfft,
+                              uint32_t test_num, double &cost)
+    {
+        cost = timed(test_num, [&] { ExternalProduct_st(res, trlwe, trgswfft, 0); });
+    }
+    void ExternalProduct_withoutFusion_test(TRLWE_ &res, const TRLWE_ &trlwe,
+                                            const TRGSWFFT_ &trgswfft, uint32_t test_num,
+                                            double &cost)
+    {
+        ExternalProduct_test(res, trlwe, trgswfft, test_num, cost);
     }
 
     // ---- batched forms (host arrays) -------------------------------------
@@ -265,30 +431,17 @@ class CuFFT_Torus {
     // (tv_stride 0) or B x N (tv_stride N)
     void pbs_batch(T *out, const uint32_t *in, const T *tv, int tv_stride, int B)
     {
-        const size_t in_b = size_t(B) * (rt_.n0() + 1) * 4, out_b = size_t(B) * (P::n + 1) * sizeof(T);
-        const size_t tv_b = (tv_stride ? size_t(B) : 1) * P::n * sizeof(T);
-        din_.reset(in_b);
-        dtv_.reset(tv_b);
-        dout_.reset(out_b);
-        din_.upload(in, in_b);
-        dtv_.upload(tv, tv_b);
-        if constexpr (kLevel == 1)
-            cufhedb_amd::check(fhe_pbs_lvl01_dev(rt_.ctx(), din_.as<uint32_t>(), dtv_.as<uint32_t>(),
-                                                 tv_stride, dout_.as<uint32_t>(), B, nullptr),
-                               "fhe_pbs_lvl01_dev");
-        else
-            cufhedb_amd::check(fhe_pbs_lvl02_dev(rt_.ctx(), din_.as<uint32_t>(), dtv_.as<uint64_t>(),
-                                                 tv_stride, dout_.as<uint64_t>(), B, nullptr),
-                               "fhe_pbs_lvl02_dev");
-        cufhedb_amd::check(fhe_ctx_sync(rt_.ctx()), "fhe_ctx_sync");
-        dout_.download(out, out_b);
+        std::lock_guard<std::mutex> lk(runtime().mutex());
+        pbs_batch_locked(out, in, tv, tv_stride, B);
     }
     // GateBootstrappingManyLUT<P, num_out> (TFHEpp gatebootstrapping.hpp:119-130):
     // B lvl0 TLWEs -> B x num_out TLWEs, tables interleaved in tv
     void pbs_many_lut_batch(T *out, const uint32_t *in, const T *tv, int tv_stride, int num_out,
                             int B)
     {
-        const size_t in_b = size_t(B) * (rt_.n0() + 1) * 4;
+        std::lock_guard<std::mutex> lk(runtime().mutex());
+        cufhedb_amd::Runtime &rt = runtime();
+        const size_t in_b = size_t(B) * (rt.n0() + 1) * 4;
         const size_t out_b = size_t(B) * num_out * (P::n + 1) * sizeof(T);
         const size_t tv_b = (tv_stride ? size_t(B) : 1) * P::n * sizeof(T);
         din_.reset(in_b);
@@ -297,20 +450,49 @@ class CuFFT_Torus {
         din_.upload(in, in_b);
         dtv_.upload(tv, tv_b);
         if constexpr (kLevel == 1)
-            cufhedb_amd::check(fhe_pbs_many_lut_lvl01_dev(rt_.ctx(), din_.as<uint32_t>(),
+            cufhedb_amd::check(fhe_pbs_many_lut_lvl01_dev(rt.ctx(), din_.as<uint32_t>(),
                                                           dtv_.as<uint32_t>(), tv_stride, num_out,
                                                           dout_.as<uint32_t>(), B, nullptr),
                                "fhe_pbs_many_lut_lvl01_dev");
         else
-            cufhedb_amd::check(fhe_pbs_many_lut_lvl02_dev(rt_.ctx(), din_.as<uint32_t>(),
+            cufhedb_amd::check(fhe_pbs_many_lut_lvl02_dev(rt.ctx(), din_.as<uint32_t>(),
                                                           dtv_.as<uint64_t>(), tv_stride, num_out,
                                                           dout_.as<uint64_t>(), B, nullptr),
                                "fhe_pbs_many_lut_lvl02_dev");
-        cufhedb_amd::check(fhe_ctx_sync(rt_.ctx()), "fhe_ctx_sync");
+        cufhedb_amd::check(fhe_ctx_sync(rt.ctx()), "fhe_ctx_sync");
         dout_.download(out, out_b);
     }
     void external_product_batch(T *out, const T *trlwe, const double *trgswfft, int B)
     {
+        std::lock_guard<std::mutex> lk(runtime().mutex());
+        external_product_locked(out, trlwe, trgswfft, B);
+    }
+
+  private:
+    void pbs_batch_locked(T *out, const uint32_t *in, const T *tv, int tv_stride, int B)
+    {
+        cufhedb_amd::Runtime &rt = runtime();
+        const size_t in_b = size_t(B) * (rt.n0() + 1) * 4, out_b = size_t(B) * (P::n + 1) * sizeof(T);
+        const size_t tv_b = (tv_stride ? size_t(B) : 1) * P::n * sizeof(T);
+        din_.reset(in_b);
+        dtv_.reset(tv_b);
+        dout_.reset(out_b);
+        din_.upload(in, in_b);
+        dtv_.upload(tv, tv_b);
+        if constexpr (kLevel == 1)
+            cufhedb_amd::check(fhe_pbs_lvl01_dev(rt.ctx(), din_.as<uint32_t>(), dtv_.as<uint32_t>(),
+                                                 tv_stride, dout_.as<uint32_t>(), B, nullptr),
+                               "fhe_pbs_lvl01_dev");
+        else
+            cufhedb_amd::check(fhe_pbs_lvl02_dev(rt.ctx(), din_.as<uint32_t>(), dtv_.as<uint64_t>(),
+                                                 tv_stride, dout_.as<uint64_t>(), B, nullptr),
+                               "fhe_pbs_lvl02_dev");
+        cufhedb_amd::check(fhe_ctx_sync(rt.ctx()), "fhe_ctx_sync");
+        dout_.download(out, out_b);
+    }
+    void external_product_locked(T *out, const T *trlwe, const double *trgswfft, int B)
+    {
+        cufhedb_amd::Runtime &rt = runtime();
         const size_t io_b = size_t(B) * 2 * P::n * sizeof(T);
         const size_t g_b = size_t(2 * P::l) * 2 * P::n * sizeof(double);
         din_.reset(io_b);
@@ -319,45 +501,171 @@ class CuFFT_Torus {
         din_.upload(trlwe, io_b);
         dtv_.upload(trgswfft, g_b);
         if constexpr (kLevel == 1)
-            cufhedb_amd::check(fhe_external_product_lvl1_dev(rt_.ctx(), din_.as<uint32_t>(),
+            cufhedb_amd::check(fhe_external_product_lvl1_dev(rt.ctx(), din_.as<uint32_t>(),
                                                              dtv_.as<double>(), dout_.as<uint32_t>(),
                                                              B, nullptr),
                                "fhe_external_product_lvl1_dev");
         else
-            cufhedb_amd::check(fhe_external_product_lvl2_dev(rt_.ctx(), din_.as<uint64_t>(),
+            cufhedb_amd::check(fhe_external_product_lvl2_dev(rt.ctx(), din_.as<uint64_t>(),
                                                              dtv_.as<double>(), dout_.as<uint64_t>(),
                                                              B, nullptr),
                                "fhe_external_product_lvl2_dev");
-        cufhedb_amd::check(fhe_ctx_sync(rt_.ctx()), "fhe_ctx_sync");
+        cufhedb_amd::check(fhe_ctx_sync(rt.ctx()), "fhe_ctx_sync");
         dout_.download(out, io_b);
     }
-
-  private:
     template <class A, class Bp>
-    void transform(A *dst, const Bp *src, int B, bool to_fd)
+    void transform_locked(A *dst, const Bp *src, int B, bool to_fd)
     {
+        cufhedb_amd::Runtime &rt = runtime();
         const size_t tb = size_t(B) * P::n * sizeof(T), fb = size_t(B) * P::n * sizeof(double);
         din_.reset(to_fd ? tb : fb);
         dout_.reset(to_fd ? fb : tb);
         din_.upload(src, to_fd ? tb : fb);
         int rc;
         if (to_fd)
-            rc = kLevel == 1 ? fhe_ifft_lvl1_dev(rt_.ctx(), din_.as<uint32_t>(), dout_.as<double>(), B, nullptr)
-                             : fhe_ifft_lvl2_dev(rt_.ctx(), din_.as<uint64_t>(), dout_.as<double>(), B, nullptr);
+            rc = kLevel == 1 ? fhe_ifft_lvl1_dev(rt.ctx(), din_.as<uint32_t>(), dout_.as<double>(), B, nullptr)
+                             : fhe_ifft_lvl2_dev(rt.ctx(), din_.as<uint64_t>(), dout_.as<double>(), B, nullptr);
         else
-            rc = kLevel == 1 ? fhe_fft_lvl1_dev(rt_.ctx(), din_.as<double>(), dout_.as<uint32_t>(), B, nullptr)
-                             : fhe_fft_lvl2_dev(rt_.ctx(), din_.as<double>(), dout_.as<uint64_t>(), B, nullptr);
+            rc = kLevel == 1 ? fhe_fft_lvl1_dev(rt.ctx(), din_.as<double>(), dout_.as<uint32_t>(), B, nullptr)
+                             : fhe_fft_lvl2_dev(rt.ctx(), din_.as<double>(), dout_.as<uint64_t>(), B, nullptr);
         cufhedb_amd::check(rc, "transform");
-        cufhedb_amd::check(fhe_ctx_sync(rt_.ctx()), "fhe_ctx_sync");
+        cufhedb_amd::check(fhe_ctx_sync(rt.ctx()), "fhe_ctx_sync");
         dout_.download(dst, to_fd ? fb : tb);
     }
+    // TFHEpp MulInFD (utils.hpp:83-95): complex product of [re | im] halves
+    static void mul_in_fd(double *res, const double *a, const double *b)
+    {
+        constexpr uint32_t h = P::n / 2;
+        for (uint32_t i = 0; i < h; i++) {
+            const double re = a[i] * b[i] - a[i + h] * b[i + h];
+            const double im = a[i] * b[i + h] + a[i + h] * b[i];
+            res[i] = re;
+            res[i + h] = im;
+        }
+    }
+    // TFHEpp DecompositionPolynomial digit d (trgsw.cpp:9-39), as a torus word
+    static T digit(T x, int d)
+    {
+        constexpr int w = int(sizeof(T) * 8);
+        constexpr T half = T(1) << (P::Bgbit - 1);
+        T offset = 0;
+        for (int i = 1; i <= int(P::l); i++) offset += half << (w - i * int(P::Bgbit));
+        offset += T(1) << (w - int(P::l) * int(P::Bgbit) - 1);
+        const T u = ((x + offset) >> (w - (d + 1) * int(P::Bgbit))) & ((T(1) << P::Bgbit) - 1);
+        return u - half;
+    }
+    template <class Fn>
+    static double timed(uint32_t test_num, Fn &&fn)
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint32_t i = 0; i < test_num; i++) fn();
+        return double(std::chrono::duration_cast<std::chrono::microseconds>(
+                          std::chrono::steady_clock::now() - t0)
+                          .count());
+    }
 
-    cufhedb_amd::Runtime &rt_;
+    cufhedb_amd::Runtime *rt_ = nullptr;
+    bool on_default_ = false;
     int num_stream_;
     cufhedb_amd::DeviceBuffer din_, dtv_, dout_;
+    std::vector<double> mult_fd_;  // PolyMul_st multiplier, FD
+    TRLWE_ last_trlwe_{};          // inputs of the last ExternalProduct_st (MoveBuf)
+    TRGSWFFT_ last_trgsw_{};
+    bool has_last_ = false;
 };
 
 }  // namespace cufft
+
+// The reference's global singletons (gatebootstrap_gpu.cuh:24-25), on the
+// default Runtime; constructing them touches no GPU.
+inline cufft::CuFFT_Torus<cufhedb_amd::lvl1param> cufftlvl1(20);
+inline cufft::CuFFT_Torus<cufhedb_amd::lvl2param> cufftlvl2(20);
+
+namespace cufhedb_amd {
+namespace detail {
+
+// Keys of a TFHEpp::EvalKey-like object (cloudkey.hpp:93-248): pointer-like
+// members with TFHEpp's names whose pointees are the TFHEpp aggregates,
+// contiguous, in the layouts fhe_load_ksk / fhe_load_bk (FHE_BK_FFT) take.
+#define CUFHEDB_AMD_HAS_MEMBER(NAME)                                                        \
+    template <class EK, class = void>                                                       \
+    struct has_##NAME : std::false_type {};                                                  \
+    template <class EK>                                                                      \
+    struct has_##NAME<EK, std::void_t<decltype(*std::declval<const EK &>().NAME)>>          \
+        : std::true_type {};
+CUFHEDB_AMD_HAS_MEMBER(iksklvl10)
+CUFHEDB_AMD_HAS_MEMBER(iksklvl20)
+CUFHEDB_AMD_HAS_MEMBER(iksklvl21)
+CUFHEDB_AMD_HAS_MEMBER(bkfftlvl01)
+CUFHEDB_AMD_HAS_MEMBER(bkfftlvl02)
+#undef CUFHEDB_AMD_HAS_MEMBER
+
+template <class K>
+inline void upload_ksk(Runtime &rt, int kind, const K &key)
+{
+    if (!key) throw Error(FHE_ERR_NOT_LOADED, "EvalKey: key-switching key " + std::to_string(kind) + " is empty");
+    if (sizeof(*key) != fhe_ksk_words(&rt.params(), kind) * sizeof(uint32_t))
+        throw Error(FHE_ERR_ARG, "EvalKey: key-switching key " + std::to_string(kind) +
+                                     " does not match the Runtime's parameters");
+    rt.load_ksk(kind, reinterpret_cast<const uint32_t *>(&*key));
+}
+template <class K>
+inline void upload_bkfft(Runtime &rt, int level, const K &key)
+{
+    if (!key) throw Error(FHE_ERR_NOT_LOADED, "EvalKey: bootstrapping key is empty");
+    const size_t want = size_t(rt.n0()) * (level == 1 ? 6 * 2 * 1024 : 8 * 2 * 2048) * sizeof(double);
+    if (sizeof(*key) != want)
+        throw Error(FHE_ERR_ARG, "EvalKey: bootstrapping key does not match the Runtime's parameters");
+    rt.load_bk(level, &*key, FHE_BK_FFT);
+}
+
+// Key-switching keys of the given kinds and bootstrapping keys of the given
+// levels on the device: the ones the Runtime lacks are uploaded once from
+// the EvalKey (the reference reads *ek.iksklvl10 / 20 / 21 on the host in
+// every call; its BKs are loaded by the caller through LoadBK).
+template <class EK>
+inline void ensure_keys(Runtime &rt, const EK &ek, std::initializer_list<int> ksks,
+                        std::initializer_list<int> bks)
+{
+    for (int kind : ksks) {
+        if (rt.has_ksk(kind)) continue;
+        if constexpr (has_iksklvl10<EK>::value && has_iksklvl20<EK>::value &&
+                      has_iksklvl21<EK>::value) {
+            if (kind == 10) upload_ksk(rt, 10, ek.iksklvl10);
+            if (kind == 20) upload_ksk(rt, 20, ek.iksklvl20);
+            if (kind == 21) upload_ksk(rt, 21, ek.iksklvl21);
+        } else {
+            throw Error(FHE_ERR_NOT_LOADED, "key-switching key " + std::to_string(kind) +
+                                                " not loaded and the EvalKey has none");
+        }
+    }
+    for (int level : bks) {
+        if (rt.has_bk(level)) continue;
+        if constexpr (has_bkfftlvl01<EK>::value && has_bkfftlvl02<EK>::value) {
+            if (level == 1)
+                upload_bkfft(rt, 1, ek.bkfftlvl01);
+            else
+                upload_bkfft(rt, 2, ek.bkfftlvl02);
+        } else {
+            throw Error(FHE_ERR_NOT_LOADED, "bootstrapping key lvl0" + std::to_string(level) +
+                                                " not loaded and the EvalKey has none");
+        }
+    }
+}
+
+// one operator call on one ciphertext: in (in_words of TI) -> out (out_words of TO)
+template <class TI, class TO, class Fn>
+inline void run_one(Runtime &rt, TO *out, size_t out_words, const TI *in, size_t in_words, Fn &&fn)
+{
+    DeviceBuffer din(in_words * sizeof(TI)), dout(out_words * sizeof(TO));
+    din.upload(in, in_words * sizeof(TI));
+    check(fn(rt.ctx(), din.get(), dout.get()), "operator");
+    check(fhe_ctx_sync(rt.ctx()), "fhe_ctx_sync");
+    dout.download(out, out_words * sizeof(TO));
+}
+
+}  // namespace detail
+}  // namespace cufhedb_amd
 
 namespace cuHEDB {
 
@@ -570,6 +878,180 @@ inline void log_rescale(cufhedb_amd::Runtime &rt, TLWELvl1 &res, const TLWELvl1 
         res.data(), tlwe.data(), int(scale_bits), 1);
 }
 
+// ---- the reference's own signatures ----------------------------------------
+// The operators of gatebootstrap_gpu.cuh:31-170, msb_gpu.cuh:57 / 175 and
+// HomCompare_gpu.cuh:14-76 with their exact parameter lists: the TFHEpp
+// EvalKey where the reference takes it (any type with TFHEpp's member names;
+// its key-switching keys and BKs are uploaded once to the default Runtime if
+// that lacks them) and the stream_id last.  They run on
+// cufhedb_amd::default_runtime(), as the reference's run on its global
+// singletons, one ciphertext per call, serialised on the context.
+
+namespace detail {
+template <class EK>
+inline void msb_or_ide(TLWELvl1 *r1, uint64_t *r2, const uint32_t *i1, const uint64_t *i2,
+                       const EK &ek, bool ide, uint32_t arg)
+{
+    auto &rt = cufhedb_amd::default_runtime();
+    std::lock_guard<std::mutex> lk(rt.mutex());
+    const int in_level = i1 ? 1 : 2, out_level = r1 ? 1 : 2;
+    cufhedb_amd::detail::ensure_keys(rt, ek, {in_level == 1 ? 10 : 20}, {out_level});
+    auto fn = [&](fhe_ctx *c, const void *in, void *out) {
+        return ide ? fhe_ide_gate_dev(c, in_level, out_level, in, out, arg, 1, nullptr)
+                   : fhe_msb_gate_dev(c, in_level, out_level, in, out, int(arg), 1, nullptr);
+    };
+    const size_t w1 = cufhedb_amd::lvl1param::n + 1, w2 = cufhedb_amd::lvl2param::n + 1;
+    if (in_level == 1) {
+        if (out_level == 1) cufhedb_amd::detail::run_one(rt, r1->data(), w1, i1, w1, fn);
+        else cufhedb_amd::detail::run_one(rt, r2, w2, i1, w1, fn);
+    } else {
+        if (out_level == 1) cufhedb_amd::detail::run_one(rt, r1->data(), w1, i2, w2, fn);
+        else cufhedb_amd::detail::run_one(rt, r2, w2, i2, w2, fn);
+    }
+}
+}  // namespace detail
+
+// MSBGateBootstrapping (gatebootstrap_gpu.cuh:31-80): lvl1 -> lvl1,
+// lvl2 -> lvl1, lvl2 -> lvl2
+template <class EK>
+inline void MSBGateBootstrapping(TLWELvl1 &res, const TLWELvl1 &tlwe, const EK &ek,
+                                 bool result_type, uint32_t /*stream_id*/)
+{
+    detail::msb_or_ide(&res, nullptr, tlwe.data(), nullptr, ek, false, result_type ? 1 : 0);
+}
+template <class EK>
+inline void MSBGateBootstrapping(TLWELvl1 &res, const TLWELvl2 &tlwe, const EK &ek,
+                                 bool result_type, uint32_t /*stream_id*/)
+{
+    detail::msb_or_ide(&res, nullptr, nullptr, tlwe.data(), ek, false, result_type ? 1 : 0);
+}
+template <class EK>
+inline void MSBGateBootstrapping(TLWELvl2 &res, const TLWELvl2 &tlwe, const EK &ek,
+                                 bool result_type, uint32_t /*stream_id*/)
+{
+    detail::msb_or_ide(nullptr, res.data(), nullptr, tlwe.data(), ek, false, result_type ? 1 : 0);
+}
+// IdeGateBootstrapping (gatebootstrap_gpu.cuh:82-115): the same three pairs;
+// for lvl2 -> lvl1 scale_bits is on the 64-bit torus, as there
+template <class EK>
+inline void IdeGateBootstrapping(TLWELvl1 &res, const TLWELvl1 &tlwe, uint32_t scale_bits,
+                                 const EK &ek, uint32_t /*stream_id*/)
+{
+    detail::msb_or_ide(&res, nullptr, tlwe.data(), nullptr, ek, true, scale_bits);
+}
+template <class EK>
+inline void IdeGateBootstrapping(TLWELvl1 &res, const TLWELvl2 &tlwe, uint32_t scale_bits,
+                                 const EK &ek, uint32_t /*stream_id*/)
+{
+    detail::msb_or_ide(&res, nullptr, nullptr, tlwe.data(), ek, true, scale_bits);
+}
+template <class EK>
+inline void IdeGateBootstrapping(TLWELvl2 &res, const TLWELvl2 &tlwe, uint32_t scale_bits,
+                                 const EK &ek, uint32_t /*stream_id*/)
+{
+    detail::msb_or_ide(nullptr, res.data(), nullptr, tlwe.data(), ek, true, scale_bits);
+}
+
+namespace detail {
+template <class EK>
+inline void gate_one(int gate, TLWELvl1 &res, const TLWELvl1 &ca, const TLWELvl1 &cb,
+                     const EK &ek, bool result_type)
+{
+    auto &rt = cufhedb_amd::default_runtime();
+    std::lock_guard<std::mutex> lk(rt.mutex());
+    cufhedb_amd::detail::ensure_keys(rt, ek, {10}, {1});
+    run_gate(rt, gate, res.data(), ca.data(), cb.data(), result_type, 1);
+}
+template <class EK, class Fn>
+inline void unary_one(TLWELvl1 &res, const TLWELvl1 &tlwe, const EK &ek, Fn &&fn)
+{
+    auto &rt = cufhedb_amd::default_runtime();
+    std::lock_guard<std::mutex> lk(rt.mutex());
+    cufhedb_amd::detail::ensure_keys(rt, ek, {10}, {1});
+    const size_t w1 = cufhedb_amd::lvl1param::n + 1;
+    cufhedb_amd::detail::run_one(rt, res.data(), w1, tlwe.data(), w1, fn);
+}
+}  // namespace detail
+
+// HomAND / HomOR (gatebootstrap_gpu.cuh:123-153)
+template <class EK>
+inline void HomAND(TLWELvl1 &res, const TLWELvl1 &ca, const TLWELvl1 &cb, const EK &ek,
+                   bool result_type, uint32_t /*stream_id*/)
+{
+    detail::gate_one(FHE_GATE_AND, res, ca, cb, ek, result_type);
+}
+template <class EK>
+inline void HomOR(TLWELvl1 &res, const TLWELvl1 &ca, const TLWELvl1 &cb, const EK &ek,
+                  bool result_type, uint32_t /*stream_id*/)
+{
+    detail::gate_one(FHE_GATE_OR, res, ca, cb, ek, result_type);
+}
+// LOG_to_ARI / log_rescale (gatebootstrap_gpu.cuh:155-170; HEDB's CPU test
+// vector, see the Runtime forms above)
+template <class EK>
+inline void LOG_to_ARI(TLWELvl1 &res, const TLWELvl1 &tlwe, const EK &ek, uint32_t /*stream_id*/)
+{
+    detail::unary_one(res, tlwe, ek, [](fhe_ctx *c, const void *in, void *out) {
+        return fhe_log_to_ari_lvl1_dev(c, static_cast<const uint32_t *>(in),
+                                       static_cast<uint32_t *>(out), 1, nullptr);
+    });
+}
+template <class EK>
+inline void log_rescale(TLWELvl1 &res, const TLWELvl1 &tlwe, uint32_t scale_bits, const EK &ek,
+                        uint32_t /*stream_id*/)
+{
+    detail::unary_one(res, tlwe, ek, [scale_bits](fhe_ctx *c, const void *in, void *out) {
+        return fhe_log_rescale_lvl1_dev(c, static_cast<const uint32_t *>(in),
+                                        static_cast<uint32_t *>(out), scale_bits, 1, nullptr);
+    });
+}
+
+// HomMSB (msb_gpu.cuh:57 lvl1, :175 lvl2)
+template <class T, size_t W, class EK>
+inline void HomMSB(TLWELvl1 &res, const std::array<T, W> &tlwe, uint32_t plain_bits, const EK &ek,
+                   bool result_type, uint32_t /*stream_id*/)
+{
+    constexpr int lvl = detail::level_of<T>();
+    auto &rt = cufhedb_amd::default_runtime();
+    std::lock_guard<std::mutex> lk(rt.mutex());
+    if (lvl == 1)
+        cufhedb_amd::detail::ensure_keys(rt, ek, {10}, {1});
+    else
+        cufhedb_amd::detail::ensure_keys(rt, ek, {10, 20, 21}, {1, 2});
+    cufhedb_amd::detail::run_one(rt, res.data(), res.size(), tlwe.data(), W,
+                                 [&](fhe_ctx *c, const void *in, void *out) {
+                                     return fhe_hom_msb_dev(c, lvl, in, static_cast<uint32_t *>(out),
+                                                            plain_bits, result_type ? 1 : 0, 1,
+                                                            nullptr);
+                                 });
+}
+
+// greater_than .. equal (HomCompare_gpu.cuh:14-76), template parameter P
+// (Lvl1 or Lvl2) as there
+#define CUFHEDB_AMD_REFCMP(NAME, OP)                                                          \
+    template <class P, class EK>                                                              \
+    inline void NAME(const std::array<typename P::T, P::k * P::n + 1> &cipher1,               \
+                     const std::array<typename P::T, P::k * P::n + 1> &cipher2, TLWELvl1 &res, \
+                     uint32_t plain_bits, const EK &ek, bool result_type,                     \
+                     uint32_t /*stream_id*/)                                                  \
+    {                                                                                         \
+        constexpr int lvl = detail::level_of<typename P::T>();                                \
+        auto &rt = cufhedb_amd::default_runtime();                                            \
+        std::lock_guard<std::mutex> lk(rt.mutex());                                           \
+        if (lvl == 1)                                                                         \
+            cufhedb_amd::detail::ensure_keys(rt, ek, {10}, {1});                              \
+        else                                                                                  \
+            cufhedb_amd::detail::ensure_keys(rt, ek, {10, 20, 21}, {1, 2});                   \
+        detail::run_compare(rt, OP, cipher1.data(), cipher2.data(), false, res.data(),        \
+                            plain_bits, result_type, 1);                                      \
+    }
+CUFHEDB_AMD_REFCMP(greater_than, FHE_CMP_GT)
+CUFHEDB_AMD_REFCMP(greater_than_equal, FHE_CMP_GE)
+CUFHEDB_AMD_REFCMP(less_than, FHE_CMP_LT)
+CUFHEDB_AMD_REFCMP(less_than_equal, FHE_CMP_LE)
+CUFHEDB_AMD_REFCMP(equal, FHE_CMP_EQ)
+#undef CUFHEDB_AMD_REFCMP
+
 }  // namespace cuHEDB
 
 // HEDB key files (src/HEDB/utils/serialize.cpp:7-320).  The reference's
@@ -577,6 +1059,17 @@ inline void log_rescale(cufhedb_amd::Runtime &rt, TLWELvl1 &res, const TLWELvl1 
 // load the same files into a Runtime's device keys.  The generate_*_key
 // archive step is fhe_keyfile_write / fhe_keyfile_write_secret.
 namespace HEDB {
+
+// HEDB/utils/types.h:4-35 names the reference's callers use (the reference
+// header gatebootstrap_gpu.cuh also puts them in the global namespace with
+// `using namespace HEDB`; a caller of this header writes that line itself)
+using Lvl1 = cufhedb_amd::lvl1param;
+using Lvl2 = cufhedb_amd::lvl2param;
+using TLWELvl1 = std::array<uint32_t, Lvl1::n + 1>;
+using TLWELvl2 = std::array<uint64_t, Lvl2::n + 1>;
+using TLWELvl0 = std::array<uint32_t, 673>;  // the reference's lvl0 (n = 672)
+constexpr bool LOGIC = true;
+constexpr bool ARITHMETIC = false;
 
 struct SecretKey {
     std::vector<uint32_t> lvl0;

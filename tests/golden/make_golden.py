@@ -12,8 +12,9 @@ integer-exact IKS vectors are uniform words defined row by row from a seed
 TFHEpp is compiled with params/128bit.hpp, so the lvl0 dimension of every
 vector here is n0 = 672.
 
-Usage:  python tests/golden/make_golden.py [many_lut]
-        (many_lut: write only pbs_many_lvl01_ref672.npz)
+Usage:  python tests/golden/make_golden.py [many_lut | lvl2]
+        (many_lut: write only pbs_many_lvl01_ref672.npz;
+         lvl2: write only pbs_lvl02_ref672.npz)
 """
 
 import mmap
@@ -216,6 +217,7 @@ def main():
                         msb_arith=F.tlwe_phase_u32(msb_arith, sk.s1),
                         ide=F.tlwe_phase_u32(ide, sk.s1))
     many_lut(ref, sk, p672, bkfft)
+    lvl2_pbs(ref, sk, p672)
     print("golden vectors written to", HERE)
 
 
@@ -241,6 +243,57 @@ def many_lut(ref, sk, p672, bkfft):
     np.savez_compressed(os.path.join(HERE, "pbs_many_lvl01_ref672.npz"), **vec)
 
 
+def z8_testvector64(lut, N=N2):
+    """Z_8 lookup table on the 64-bit torus: N/8 = 256 entries per value,
+    half a box of offset (the lvl02 form of the lvl01 tables above)."""
+    j = np.arange(N)
+    box = (j + N // 16) // (N // 8)
+    vals = np.array(lut, np.uint64)[box % 8] << np.uint64(60)
+    return np.where(box == 8, np.uint64(0) - vals, vals).astype(np.uint64)
+
+
+def lvl2_pbs(ref, sk, p672):
+    """GateBootstrappingTLWE2TLWEFFT<lvl02param> (gatebootstrapping.cpp:14-23)
+    and GateBootstrappingManyLUT<lvl02param, 2> (gatebootstrapping.hpp:119-130)
+    on a seeded lvl02 key (BK seed 672003), Z_8 plaintexts m * 2^28, random
+    tables; phases with the lvl2 key."""
+    P = O.P
+    bk2 = F.keygen_bk_lvl02(sk, 672003)
+    bkfft = np.zeros(bk2.shape)
+    for i in range(bk2.shape[0]):
+        ref.ref_trgsw_to_fft_lvl2(P(bkfft[i]), P(np.ascontiguousarray(bk2[i])))
+    rng = np.random.default_rng(2002)
+    B = 8
+    m = rng.integers(0, 8, B)
+    lut = rng.integers(0, 8, 8)
+    tv = z8_testvector64(lut)
+    c0 = F.tlwe_encrypt_u32(672400, 0, sk.s0, (m << 28).astype(np.uint32), p672.alpha0)
+    out = np.zeros((B, N2 + 1), np.uint64)
+    for b in range(B):
+        ref.ref_pbs_lvl02(P(out[b]), P(c0[b]), P(bkfft), P(tv))
+    # two tables, Z_4 plaintexts m * 2^29 (the stride-2 modulus switch)
+    mm = rng.integers(0, 4, 4)
+    luts = rng.integers(0, 16, (2, 4))
+    tvm = F.many_lut_testvector(luts.astype(np.uint64) << np.uint64(60), 2, N2)
+    cm = F.tlwe_encrypt_u32(672401, 0, sk.s0, (mm << 29).astype(np.uint32), p672.alpha0)
+    outm = np.zeros((4, 2, N2 + 1), np.uint64)
+    for b in range(4):
+        assert ref.ref_pbs_many_lvl02(P(outm[b]), P(cm[b]), P(bkfft), P(tvm), 2) == 0
+    np.savez_compressed(os.path.join(HERE, "pbs_lvl02_ref672.npz"), sk_seed=672001,
+                        bk_seed=672003, m=m, lut=lut, tv=tv, c0=c0,
+                        out_phase=O.tlwe_phase64(out, sk.s2), mm=mm, luts=luts, tvm=tvm, cm=cm,
+                        outm_phase=O.tlwe_phase64(outm.reshape(-1, N2 + 1), sk.s2).reshape(4, 2))
+
+
+def lvl2_only():
+    """Write only the lvl02 vectors (the other files are unchanged)."""
+    ref = O.ref()
+    if ref is None:
+        sys.exit("oracle/_ref/libtfhepp_ref.so is missing: run `make ref` first")
+    p672 = F.preset(F.PRESET_REF672)
+    lvl2_pbs(ref, F.SecretKeys(p672, 672001), p672)
+
+
 def many_lut_only():
     """Write only the many-table vectors (the other files are unchanged)."""
     ref = O.ref()
@@ -258,5 +311,7 @@ def many_lut_only():
 if __name__ == "__main__":
     if sys.argv[1:] == ["many_lut"]:
         many_lut_only()
+    elif sys.argv[1:] == ["lvl2"]:
+        lvl2_only()
     else:
         main()

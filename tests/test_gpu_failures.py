@@ -1,5 +1,8 @@
 """GPU: decryption-failure count over 10^6 lvl1 programmable bootstraps.
 
+Both lvl1 variants run it: the FP64 path and the split-MFMA path (fp16
+matrix-core forward transforms, key in three integer limbs).
+
 Workload: the batched Z_8 lookup-table PBS of the benchmark (config preset
 n0 = 636, alpha0 = 2^-15, batch 8192), 123 batches = 1,007,616 PBS with fresh
 plaintexts, encryption seeds and a fresh 8-entry LUT per batch.  Each output
@@ -27,8 +30,20 @@ def z8_testvector(lut):
     return (vals % (1 << 32)).astype(np.uint32)
 
 
-def test_no_decryption_failures_in_1e6_pbs(keys636, ctx636):
-    k, ctx = keys636, ctx636
+@pytest.fixture(params=["fp64", "mfma_split"])
+def variant_ctx(request, keys636, ctx636):
+    if request.param == "fp64":
+        yield ctx636
+        return
+    ctx = F.Context(keys636.params)
+    ctx.load_bk(1, keys636.bk, F.BK_INT)
+    ctx.set_pbs_variant(F.PBS_MFMA_SPLIT)
+    yield ctx
+    ctx.close()
+
+
+def test_no_decryption_failures_in_1e6_pbs(keys636, variant_ctx):
+    k, ctx = keys636, variant_ctx
     B, batches = 8192, 123
     rng = np.random.default_rng(1000003)
     d_in = F.DeviceBuffer(B * (k.n0 + 1) * 4)

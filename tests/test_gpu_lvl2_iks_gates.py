@@ -6,10 +6,17 @@ Bars:
     (TFHEpp/test/polymul.cpp:58-67).
   * lvl2 external product: within 2^32 of the oracle on uniform inputs and
     decrypt-exact on encryptions.
-  * lvl2 PBS: decrypt-exact; phase within 2^-6 of the oracle's.
+  * lvl2 PBS: decrypt-exact; every output's phase within 2^40 of the
+    oracle's (the GPU's output noise is 2^36.2 in 2^-64 units, the oracle's
+    2^35.65, so their per-sample gap has std ~2^36.7: 2^40 is ~10 sigma; a
+    precision loss of a few bits would show); the noise statistics
+    themselves are in test_gpu_noise.py.
   * IKS lvl10 / lvl20 / lvl21: bit-exact against the oracle, decrypt-exact.
   * gates (NAND, AND, OR, NOR, XNOR, MSB, Ide): truth tables exact.
 """
+
+import concurrent.futures as cf
+import os
 
 import numpy as np
 import pytest
@@ -22,6 +29,7 @@ pytestmark = pytest.mark.gpu
 
 N2 = 2048
 D60 = 1 << 60
+PHASE_TOL2 = 1 << 40
 MU = 1 << 29
 
 
@@ -87,9 +95,12 @@ def test_gpu_pbs_lvl02(lvl2, rng):
     dec = np.round(ph.view(np.int64) / D60).astype(np.int64) % 16
     assert np.array_equal(dec, lut[m])
     bkfft2 = O.bk_to_fft64(bk2)
-    want = O.pbs64(c0[0], bkfft2, tv)
-    dphi = int(O.tdiff64(ph[0], O.tlwe_phase64(want, k.sk.s2)[0]))
-    assert abs(dphi) <= 1 << 58  # 2^-6 of the torus
+    threads = min(16, os.cpu_count() or 1)
+    with cf.ThreadPoolExecutor(threads) as ex:
+        want = np.stack(list(ex.map(lambda b: O.pbs64(c0[b], bkfft2, tv), range(B))))
+    dphi = O.tdiff64(ph, O.tlwe_phase64(want, k.sk.s2))
+    print(f"lvl2 PBS GPU - oracle phase gaps over {B}: max 2^{np.log2(np.max(np.abs(dphi))):.1f}")
+    assert np.max(np.abs(dphi)) <= PHASE_TOL2
     acc = ctx.blind_rotate_lvl02(c0[:2], tv)
     assert np.array_equal(ctx.sample_extract_lvl2(acc, 0), out[:2])
     # the same key uploaded in TFHEpp's FD form (BootstrappingKeyFFT<lvl02param>)
@@ -97,7 +108,7 @@ def test_gpu_pbs_lvl02(lvl2, rng):
     ctx2.load_bk(2, bkfft2, F.BK_FFT)
     ph2 = O.tlwe_phase64(ctx2.pbs_lvl02(c0, tv), k.sk.s2)
     assert np.array_equal(np.round(ph2.view(np.int64) / D60).astype(np.int64) % 16, lut[m])
-    assert np.max(np.abs(O.tdiff64(ph2, ph))) <= 1 << 58
+    assert np.max(np.abs(O.tdiff64(ph2, ph))) <= PHASE_TOL2
     ctx2.close()
 
 

@@ -129,6 +129,38 @@ def test_golden_pbs_many_lut_lvl01_ref672():
                 assert int(np.round(O.tdiff32(ph[k], 0) / (1 << 28))) % 16 == luts[k][m[b]]
 
 
+# lvl2 (64-bit torus): output phase-noise std 2^35.65 (2^-64 units) for the
+# reference's operation order, which the oracle follows; two FP64
+# implementations then differ per sample by ~sqrt(2) x that, 2^36.15, so 2^40
+# is a ~14 sigma bound, 2^20 below the Z_8 decision margin (2^59).
+PHASE_TOL2 = 1 << 40
+
+
+def test_golden_pbs_lvl02_ref672():
+    """GateBootstrappingTLWE2TLWEFFT<lvl02param> and GateBootstrappingManyLUT
+    <lvl02param, 2> of the compiled reference against the oracle on the same
+    seeded key and inputs."""
+    g = load("pbs_lvl02_ref672.npz")
+    sk = F.SecretKeys(F.preset(F.PRESET_REF672), int(g["sk_seed"]))
+    bkfft = O.bk_to_fft64(F.keygen_bk_lvl02(sk, int(g["bk_seed"])))
+    gaps = []
+    for b in range(len(g["m"])):
+        ph = O.tlwe_phase64(O.pbs64(g["c0"][b], bkfft, g["tv"]), sk.s2)[0]
+        gap = int(O.tdiff64(ph, g["out_phase"][b]))
+        gaps.append(gap)
+        assert abs(gap) <= PHASE_TOL2, (b, gap)
+        assert (int(ph) + (1 << 59)) % (1 << 64) >> 60 == g["lut"][g["m"][b]]
+    for b in range(len(g["mm"])):
+        out = O.pbs_many64(g["cm"][b], bkfft, g["tvm"], 2)
+        ph = O.tlwe_phase64(out, sk.s2)
+        for k in range(2):
+            gap = int(O.tdiff64(ph[k], g["outm_phase"][b, k]))
+            gaps.append(gap)
+            assert abs(gap) <= PHASE_TOL2, (b, k, gap)
+            assert (int(ph[k]) + (1 << 59)) % (1 << 64) >> 60 == g["luts"][k][g["mm"][b]]
+    print(f"lvl02 oracle - reference phase gaps: max 2^{np.log2(max(map(abs, gaps)) + 1):.1f}")
+
+
 def test_many_lut_is_rounded_single_lut():
     """The identity the device path is built on: the K-table bootstrap's
     accumulator equals the single-table blind rotation of the lvl0 words
