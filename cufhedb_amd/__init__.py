@@ -101,6 +101,8 @@ _SIGS = {
     "fhe_ctx_stream": (_vp, [_vp]),
     "fhe_load_bk": (_i, [_vp, _i, _vp, _i]),
     "fhe_load_ksk": (_i, [_vp, _i, _vp]),
+    "fhe_bk_loaded": (_i, [_vp, _i]),
+    "fhe_ksk_loaded": (_i, [_vp, _i]),
     "fhe_keygen_ksk_dev": (_i, [_vp, _i, _u64, _vp, _vp, _vp]),
     "fhe_set_pbs_variant": (_i, [_vp, _i]),
     "fhe_pbs_lvl01_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp]),
