@@ -122,6 +122,7 @@ _SIGS = {
     "fhe_fft_lvl2_dev": (_i, [_vp, _vp, _vp, _i, _vp]),
     "fhe_measure_peaks": (_i, [_vp, _vp, _vp]),
     "fhe_gate_lvl1_dev": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp]),
+    "fhe_gate_lvl1_noalloc_dev": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _vp]),
     "fhe_gate_rescale_lvl1_dev": (_i, [_vp, _i, _vp, _vp, _vp, _u32, _i, _vp]),
     "fhe_msb_gate_lvl1_dev": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
     "fhe_ide_gate_lvl1_dev": (_i, [_vp, _vp, _vp, _u32, _i, _vp]),

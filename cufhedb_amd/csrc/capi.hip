@@ -141,6 +141,11 @@ int fhe_ctx_create(const fhe_params *p, int device, fhe_ctx **out)
         fhe_ctx_destroy(ctx);
         return hip_fail(e, "fhe_ctx_create");
     }
+    // this device's fixed scratch of the allocation-free entry point
+    if (int rc = noalloc_scratch_init(device)) {
+        fhe_ctx_destroy(ctx);
+        return rc;
+    }
     *out = ctx;
     return FHE_OK;
 }

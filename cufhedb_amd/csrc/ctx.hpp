@@ -170,6 +170,10 @@ struct GateScratch {
 int gate_lvl1(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint32_t *b_dev,
               uint32_t *out_dev, int result_type, int B, hipStream_t s, const GateScratch *fixed);
 
+// device address of the allocation-free entry point's fixed scratch on
+// `device` (current), taken once per device (noalloc.hip)
+int noalloc_scratch_init(int device);
+
 inline int ks_target_n(const fhe_ctx *ctx, int kind) { return kind == 21 ? Lvl1::N : int(ctx->p.n0); }
 inline int ks_domain_words(int kind) { return kind == 10 ? Lvl1::N + 1 : Lvl2::N + 1; }
 

@@ -238,6 +238,16 @@ enum fhe_result_type { FHE_ARITHMETIC = 0, FHE_LOGIC = 1 };
  * HEDB/utils/types.h:4-11; NAND/NOR/XNOR are always LOGIC. */
 int fhe_gate_lvl1_dev(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint32_t *b_dev,
                       uint32_t *out_dev, int result_type, int B, void *stream);
+/* The same gate bootstrap on a fixed device scratch: allocates nothing
+ * (no hipMalloc / hipFree, no allocating library call).  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * B <= 2048 (FHE_ERR_ARG above, before any launch); the scratch is one
+ * 16.8 MB array per GPU, held until the process ends, shared by every call
+ * on that GPU: calls that overlap on one GPU (two streams or host threads)
+ * race.  The lvl1 key's kernel layout must exist (fhe_load_bk builds it). */
+int fhe_gate_lvl1_noalloc_dev(fhe_ctx *ctx, int gate, const uint32_t *a_dev, const uint32_t *b_dev,
+                              uint32_t *out_dev, int result_type, int B, void *stream);
 /* AND / OR of two LOGIC lvl1 TLWEs with the result already at the scale
  * log_rescale gives it: 0 or 2^scale_bits (scale_bits in [1, 31]).  No
  * reference counterpart as one call: it decrypts as
