@@ -464,6 +464,44 @@ class Context:
     def set_pbs_variant(self, variant):
         check(lib().fhe_set_pbs_variant(self.h, variant))
 
+    # ---- per-device tuning cache (a JSON file of fhe_tune_get tables)
+    def _tune_key(self):
+        p = self.params
+        return f"{device_name()}|n0={p.n0}"
+
+    def save_tuning(self, path):
+        """Merge this context's tuned tables into the JSON cache at `path`,
+        under the device's name and the parameter set (n0): a later process
+        on the same kind of device restores them with load_tuning instead of
+        timing the candidates again."""
+        import json
+        import os
+        cache = {}
+        if os.path.exists(path):
+            with open(path) as f:
+                cache = json.load(f)
+        cache[self._tune_key()] = {str(k): self.tune_table(knob=k) for k in self.TUNE_CANDIDATES}
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+
+    def load_tuning(self, path):
+        """Restore the tables saved for this device and parameter set; returns
+        False (tables unchanged) when the cache has none."""
+        import json
+        import os
+        if not os.path.exists(path):
+            return False
+        with open(path) as f:
+            entry = json.load(f).get(self._tune_key())
+        if entry is None:
+            return False
+        for k, table in entry.items():
+            if int(k) in self.TUNE_CANDIDATES:
+                self.set_tune_table([(int(b), int(v)) for b, v in table], knob=int(k))
+        return True
+
     # ---- host-array convenience forms (synchronous)
     def pbs_lvl01(self, tlwe0, tv):
         tlwe0 = np.ascontiguousarray(tlwe0, np.uint32)

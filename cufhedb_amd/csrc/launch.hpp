@@ -18,6 +18,24 @@ hipError_t launch_bk1_from_fft(const double *bk_fft, cd *bk_dev, int n0, hipStre
 hipError_t launch_blind_rotate1(const uint32_t *in, int n0, const cd *bk, const uint32_t *tv,
                                 int tv_stride, uint32_t *out, bool sample_extract, int B,
                                 hipStream_t s);
+// One job of the persistent lvl1 bootstrap: count lvl0 TLWEs at in, one
+// test vector, count lvl1 TLWEs to out; first = sum of the previous jobs'
+// counts.
+struct Pbs1Job {
+    const uint32_t *in;
+    const uint32_t *tv;
+    uint32_t *out;
+    int first;
+    int count;
+};
+// every ciphertext of njobs jobs (total in all) through one persistent
+// launch; cursor (1 int) and done (njobs ints) are zeroed here on stream s;
+// stamp[j] receives job j's completion time on the device clock
+hipError_t launch_blind_rotate1_jobs(const Pbs1Job *jobs, int njobs, int total, int n0,
+                                     const cd *bk, int *cursor, int *done, uint64_t *stamp,
+                                     hipStream_t s);
+// the device clock (wall_clock64) now, into *t, on stream s
+hipError_t launch_clock_stamp(uint64_t *t, hipStream_t s);
 hipError_t launch_trgsw1_from_fft(const double *trgswfft, cd *trgsw_dev, hipStream_t s);
 hipError_t launch_extprod1(const uint32_t *trlwe, const cd *trgsw_dev, uint32_t *out, int B,
                            hipStream_t s);

@@ -16,6 +16,8 @@
 //   SampleExtractIndex     thirdparty/TFHEpp/src/trlwe.cpp:109-118
 //   GPU counterpart        src/cuHEDB/fft_gpu/BlindRotate_gpu.cuh:401-531
 // The a-bar == 0 skip of the reference is kept (it is wave-uniform here).
+#include <algorithm>
+
 #include "fft64.hpp"
 #include "launch.hpp"
 #include "params.hpp"
@@ -85,22 +87,25 @@ __device__ __forceinline__ void load_row(const cd *row, int lane, cd b0[8], cd b
     }
 }
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_blind_rotate1(const uint32_t *__restrict__ in, int n0,
-                                                      const cd *__restrict__ bk,
-                                                      const uint32_t *__restrict__ tv,
-                                                      int tv_stride, uint32_t *__restrict__ out,
-                                                      int sample_extract, int B)
-{
-    __shared__ uint32_t acc[2 * N1];
-    __shared__ cd tile[kFftLdsElems];
-    __shared__ uint4 planes[Lvl1::l][64];
-    const int lane = threadIdx.x;
-    const int ct = blockIdx.x;
-    if (ct >= B) return;
-    const uint32_t *c0 = in + size_t(ct) * (n0 + 1);
-    const uint32_t *tvp = tv + size_t(ct) * tv_stride;
-    const Twiddles1024 tw = make_twiddles1024(lane);
+// LDS of one blind rotation: the accumulator, the transform tile and the
+// digit planes (20 KB: two waves per SIMD)
+struct Br1Lds {
+    uint32_t acc[2 * N1];
+    cd tile[kFftLdsElems];
+    uint4 planes[Lvl1::l][64];
+};
 
+// The blind rotation of one lvl0 ciphertext c0 by the wave: acc = (0,
+// X^bbar tv), n0 CMUX steps, then sample extraction (or the TRLWE) to out.
+__device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c0, int n0,
+                                                  const cd *__restrict__ bk,
+                                                  const uint32_t *__restrict__ tvp,
+                                                  uint32_t *__restrict__ out, int sample_extract,
+                                                  Br1Lds &lds, int lane, const Twiddles1024 &tw)
+{
+    uint32_t *acc = lds.acc;
+    cd *tile = lds.tile;
+    auto &planes = lds.planes;
     // acc = (0, X^bbar * tv), bbar = 2N - (b >> 21)   (no rounding)
     const uint32_t bbar = 2u * N1 - (c0[n0] >> (32 - 1 - Lvl1::nbit));
 #pragma unroll
@@ -184,17 +189,68 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     }
 
     if (sample_extract) {
-        uint32_t *o = out + size_t(ct) * (N1 + 1);
 #pragma unroll
         for (int k = 0; k < N1 / 64; k++) {
             const int n = lane + 64 * k;
-            o[n] = n == 0 ? acc[0] : 0u - acc[N1 - n];
+            out[n] = n == 0 ? acc[0] : 0u - acc[N1 - n];
         }
-        if (lane == 0) o[N1] = acc[N1];
+        if (lane == 0) out[N1] = acc[N1];
     } else {
-        uint32_t *o = out + size_t(ct) * 2 * N1;
 #pragma unroll
-        for (int k = 0; k < 2 * N1 / 64; k++) o[lane + 64 * k] = acc[lane + 64 * k];
+        for (int k = 0; k < 2 * N1 / 64; k++) out[lane + 64 * k] = acc[lane + 64 * k];
+    }
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_blind_rotate1(const uint32_t *__restrict__ in, int n0,
+                                                      const cd *__restrict__ bk,
+                                                      const uint32_t *__restrict__ tv,
+                                                      int tv_stride, uint32_t *__restrict__ out,
+                                                      int sample_extract, int B)
+{
+    __shared__ Br1Lds lds;
+    const int lane = threadIdx.x;
+    const int ct = blockIdx.x;
+    if (ct >= B) return;
+    const Twiddles1024 tw = make_twiddles1024(lane);
+    blind_rotate1_one(in + size_t(ct) * (n0 + 1), n0, bk, tv + size_t(ct) * tv_stride,
+                      out + size_t(ct) * (sample_extract ? N1 + 1 : 2 * N1), sample_extract, lds,
+                      lane, tw);
+}
+
+// Persistent form over a table of jobs (each a batch of lvl0 ciphertexts
+// with one test vector and its own output buffer): every wave takes the next
+// ciphertext of the concatenated jobs from a device cursor until all are
+// done, so any number of requests of any sizes runs as one launch with the
+// key resident, and no inputs are gathered or test vectors replicated.  A
+// job's last finished ciphertext records the device clock into stamp[job]
+// (its completion time).  cursor and done[] must be zero at launch.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_blind_rotate1_jobs(
+    const Pbs1Job *__restrict__ jobs, int njobs, int total, int n0, const cd *__restrict__ bk,
+    int *__restrict__ cursor, int *__restrict__ done, uint64_t *__restrict__ stamp)
+{
+    __shared__ Br1Lds lds;
+    const int lane = threadIdx.x;
+    const Twiddles1024 tw = make_twiddles1024(lane);
+    for (;;) {
+        int idx = 0;
+        if (lane == 0) idx = atomicAdd(cursor, 1);
+        idx = __builtin_amdgcn_readfirstlane(idx);
+        if (idx >= total) break;
+        // the job holding ciphertext idx: the last with first <= idx
+        int lo = 0, hi = njobs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].first <= idx) lo = mid;
+            else hi = mid - 1;
+        }
+        const Pbs1Job j = jobs[lo];
+        const int ct = idx - j.first;
+        blind_rotate1_one(j.in + size_t(ct) * (n0 + 1), n0, bk, j.tv,
+                          j.out + size_t(ct) * (N1 + 1), 1, lds, lane, tw);
+        // this ciphertext's outputs are written; the job's last one stamps it
+        __threadfence();
+        if (lane == 0 && atomicAdd(done + lo, 1) == j.count - 1) stamp[lo] = wall_clock64();
+        __syncthreads();  // the next ciphertext rewrites the accumulator
     }
 }
 
@@ -375,6 +431,42 @@ hipError_t launch_blind_rotate1(const uint32_t *in, int n0, const cd *bk, const 
     if (B <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_blind_rotate1, dim3(B), dim3(64), 0, s, in, n0, bk, tv, tv_stride, out,
                        int(sample_extract), B);
+    return hipGetLastError();
+}
+
+namespace {
+__global__ void k_zero_ints(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+__global__ void k_clock_stamp(uint64_t *t) { *t = wall_clock64(); }
+}  // namespace
+
+hipError_t launch_clock_stamp(uint64_t *t, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_clock_stamp, dim3(1), dim3(1), 0, s, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_blind_rotate1_jobs(const Pbs1Job *jobs, int njobs, int total, int n0,
+                                     const cd *bk, int *cursor, int *done, uint64_t *stamp,
+                                     hipStream_t s)
+{
+    if (total <= 0 || njobs <= 0) return hipSuccess;
+    // zero-filled by kernels, not hipMemsetAsync, so the sequence stays
+    // capturable into a graph
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(1), 0, s, cursor, 1);
+    hipLaunchKernelGGL(k_zero_ints, dim3(unsigned((njobs + 255) / 256)), dim3(256), 0, s, done,
+                       njobs);
+    // as many waves as stay resident (two per SIMD), at most one per ciphertext
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    const int waves = std::min(total, cus * 4 * 2);
+    hipLaunchKernelGGL(k_blind_rotate1_jobs, dim3(unsigned(waves)), dim3(64), 0, s, jobs, njobs,
+                       total, n0, bk, cursor, done, stamp);
     return hipGetLastError();
 }
 

@@ -7,12 +7,17 @@
 // ciphertext per call (src/cuHEDB/gatebootstrap_gpu.cuh:17-25,
 // test/HE3_Query/TPCH_Q6.cu:179-193).  A request of a few thousand rows
 // fills only part of the 256 CUs (the lvl1 kernel holds 8192 ciphertexts at
-// once), so here a flush merges consecutive requests of one kind into
-// launches of up to max_rows rows: their inputs are gathered into device
-// staging buffers, the per-request test vector / threshold is replicated per
-// row (the kernels take one per ciphertext), one batched call runs, and the
-// results are scattered back.  A request of more than max_rows rows runs
-// alone, straight on its own buffers.  lvl1 launches go to the context's
+// once).  Here a flush serves every pending lvl1 request with one persistent
+// launch (k_blind_rotate1_jobs): a table of jobs (each request's input,
+// test vector and output buffers) is handed to resident waves that take the
+// next ciphertext of any request from a device cursor, with the key
+// resident, no inputs gathered and no test vectors replicated; the kernel
+// stamps each request's completion on the device clock.  lvl2 comparisons
+// are merged: consecutive requests of one kind (and op / width / result
+// type) into launches of up to max_rows rows, their inputs gathered into
+// device staging buffers, the per-request threshold replicated per row,
+// one batched call, the results scattered back; a comparison of more than
+// max_rows rows runs alone on its own buffers.  lvl1 launches go to the context's
 // stream and lvl2 comparisons to a second stream of the scheduler (default
 // priority), so a lvl1 batch and a lvl2 comparison share the CUs (each
 // kind in submission order; the second stream starts after the work
@@ -22,9 +27,11 @@
 // whose lazily built layouts are built before the streams fork.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "ctx.hpp"
+#include "launch.hpp"
 
 namespace fhe {
 namespace {
@@ -58,6 +65,30 @@ struct Req {
     hipEvent_t start = nullptr, done = nullptr;  // start: the flush that serves it began
     bool flushed = false;
     bool failed = false;  // its flush stopped on an error before its result was complete
+    int flush = -1;       // lvl1: the flush record holding its completion stamp
+    int job = -1;         //       and its job index there
+};
+
+// Device memory of one flush's persistent lvl1 launch: the job table, the
+// cursor and per-job completion counters, the flush's start time and each
+// job's completion time (device clock); kept until the scheduler ends.
+struct FlushRec {
+    void *mem = nullptr;
+    Pbs1Job *host = nullptr;  // pinned host copy of the job table (async upload)
+    int njobs = 0;
+    Pbs1Job *jobs() const { return static_cast<Pbs1Job *>(mem); }
+    int *cursor() const { return reinterpret_cast<int *>(jobs() + njobs); }
+    int *done() const { return cursor() + 1; }
+    uint64_t *stamps() const  // [0] = start, [1 + j] = job j's completion
+    {
+        const size_t off = (size_t(njobs) * sizeof(Pbs1Job) + size_t(njobs + 1) * sizeof(int) + 7) & ~size_t(7);
+        return reinterpret_cast<uint64_t *>(static_cast<char *>(mem) + off);
+    }
+    static size_t bytes(int n)
+    {
+        return ((size_t(n) * sizeof(Pbs1Job) + size_t(n + 1) * sizeof(int) + 7) & ~size_t(7)) +
+               size_t(n + 1) * sizeof(uint64_t);
+    }
 };
 
 #define SCHED_TRY(expr)                    \
@@ -76,12 +107,13 @@ struct fhe_sched {
     int max_rows = 0;
     uint64_t next_ticket = 1;
     std::vector<fhe::Req> reqs;  // submitted, not yet waited for
-    // staging, allocated on first use of each kind
-    uint32_t *in1 = nullptr, *tv1 = nullptr, *out1 = nullptr;
+    // comparison staging, allocated on first use
     uint64_t *a2 = nullptr, *b2 = nullptr;
     uint32_t *outc = nullptr;
     hipStream_t s2 = nullptr;  // lvl2 comparisons
     hipEvent_t fork = nullptr, join = nullptr;
+    std::vector<fhe::FlushRec> flushes;
+    int clock_khz = 0;  // device clock rate of the completion stamps
 };
 
 using namespace fhe;
@@ -91,11 +123,6 @@ namespace {
 int ensure_staging(fhe_sched *s, int kind)
 {
     const size_t R = size_t(s->max_rows);
-    if (kind == FHE_SCHED_PBS1 && !s->in1) {
-        HIP_TRY(hipMalloc(&s->in1, R * (s->ctx->p.n0 + 1) * 4), "scheduler staging");
-        HIP_TRY(hipMalloc(&s->tv1, R * Lvl1::N * 4), "scheduler staging");
-        HIP_TRY(hipMalloc(&s->out1, R * W1 * 4), "scheduler staging");
-    }
     if (kind == FHE_SCHED_CMP2 && !s->a2) {
         HIP_TRY(hipMalloc(&s->a2, R * W2 * 8), "scheduler staging");
         HIP_TRY(hipMalloc(&s->b2, R * W2 * 8), "scheduler staging");
@@ -111,42 +138,24 @@ bool same_group(const Req &a, const Req &b)
             (a.op == b.op && a.bits == b.bits && a.result_type == b.result_type));
 }
 
-// one batched call over rows [0, rows) of the staging buffers
+// one batched comparison over rows [0, rows) of the staging buffers
 int run_group(fhe_sched *s, const std::vector<Req *> &g, int rows, hipStream_t st)
 {
     fhe_ctx *ctx = s->ctx;
-    const int kind = g[0]->kind;
-    SCHED_TRY(ensure_staging(s, kind));
-    const size_t w0 = size_t(ctx->p.n0) + 1;
+    SCHED_TRY(ensure_staging(s, FHE_SCHED_CMP2));
     size_t off = 0;
     for (Req *r : g) {
-        if (kind == FHE_SCHED_PBS1) {
-            HIP_TRY(hipMemcpyAsync(s->in1 + off * w0, r->in, size_t(r->B) * w0 * 4,
-                                   hipMemcpyDeviceToDevice, st),
-                    "scheduler gather");
-            HIP_TRY(replicate(s->tv1 + off * Lvl1::N, static_cast<const uint32_t *>(r->aux),
-                              Lvl1::N, r->B, st),
-                    "scheduler test vectors");
-        } else {
-            HIP_TRY(hipMemcpyAsync(s->a2 + off * W2, r->in, size_t(r->B) * W2 * 8,
-                                   hipMemcpyDeviceToDevice, st),
-                    "scheduler gather");
-            HIP_TRY(replicate(s->b2 + off * W2, static_cast<const uint64_t *>(r->aux), W2, r->B,
-                              st),
-                    "scheduler thresholds");
-        }
+        HIP_TRY(hipMemcpyAsync(s->a2 + off * W2, r->in, size_t(r->B) * W2 * 8,
+                               hipMemcpyDeviceToDevice, st),
+                "scheduler gather");
+        HIP_TRY(replicate(s->b2 + off * W2, static_cast<const uint64_t *>(r->aux), W2, r->B, st),
+                "scheduler thresholds");
         off += size_t(r->B);
     }
-    uint32_t *res;
-    if (kind == FHE_SCHED_PBS1) {
-        SCHED_TRY(fhe_pbs_lvl01_dev(ctx, s->in1, s->tv1, Lvl1::N, s->out1, rows, st));
-        res = s->out1;
-    } else {
-        const Req &r0 = *g[0];
-        SCHED_TRY(fhe_compare_dev(ctx, 2, r0.op, s->a2, s->b2, 0, s->outc, r0.bits,
-                                   r0.result_type, rows, st));
-        res = s->outc;
-    }
+    const Req &r0 = *g[0];
+    SCHED_TRY(fhe_compare_dev(ctx, 2, r0.op, s->a2, s->b2, 0, s->outc, r0.bits, r0.result_type,
+                               rows, st));
+    uint32_t *res = s->outc;
     off = 0;
     for (Req *r : g) {
         HIP_TRY(hipMemcpyAsync(r->out, res + off * W1, size_t(r->B) * W1 * 4,
@@ -158,29 +167,65 @@ int run_group(fhe_sched *s, const std::vector<Req *> &g, int rows, hipStream_t s
     return FHE_OK;
 }
 
-// a request too large for the staging buffers runs on its own buffers
+// a comparison too large for the staging buffers runs on its own buffers
 int run_alone(fhe_sched *s, Req &r, hipStream_t st)
 {
-    fhe_ctx *ctx = s->ctx;
-    if (r.kind == FHE_SCHED_PBS1)
-        SCHED_TRY(fhe_pbs_lvl01_dev(ctx, static_cast<const uint32_t *>(r.in),
-                                     static_cast<const uint32_t *>(r.aux), 0, r.out, r.B, st));
-    else
-        SCHED_TRY(fhe_compare_dev(ctx, 2, r.op, r.in, r.aux, 1, r.out, r.bits, r.result_type,
-                                   r.B, st));
+    SCHED_TRY(fhe_compare_dev(s->ctx, 2, r.op, r.in, r.aux, 1, r.out, r.bits, r.result_type, r.B,
+                               st));
     HIP_TRY(hipEventRecord(r.done, st), "hipEventRecord");
+    return FHE_OK;
+}
+
+// every pending lvl1 request through one persistent launch on st1
+int serve_pbs1(fhe_sched *s, const std::vector<Req *> &reqs, hipStream_t st1)
+{
+    fhe_ctx *ctx = s->ctx;
+    if (reqs.empty()) return FHE_OK;
+    if (!ctx->bk1) {
+        set_error("fhe_sched_flush: lvl01 bootstrapping key not loaded");
+        return FHE_ERR_NOT_LOADED;
+    }
+    std::vector<Pbs1Job> jobs;
+    int total = 0;
+    for (Req *r : reqs) {
+        if (r->B == 0) continue;
+        jobs.push_back(Pbs1Job{static_cast<const uint32_t *>(r->in),
+                               static_cast<const uint32_t *>(r->aux), r->out, total, r->B});
+        r->job = int(jobs.size()) - 1;
+        total += r->B;
+    }
+    FlushRec f;
+    f.njobs = int(jobs.size());
+    HIP_TRY(hipMalloc(&f.mem, FlushRec::bytes(f.njobs)), "scheduler job table");
+    s->flushes.push_back(f);
+    FlushRec &fr = s->flushes.back();
+    const int fi = int(s->flushes.size()) - 1;
+    for (Req *r : reqs) r->flush = fi;
+    HIP_TRY(launch_clock_stamp(fr.stamps(), st1), "scheduler start stamp");
+    if (fr.njobs > 0) {
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&fr.host), jobs.size() * sizeof(Pbs1Job),
+                              hipHostMallocDefault),
+                "scheduler job table");
+        std::copy(jobs.begin(), jobs.end(), fr.host);
+        HIP_TRY(hipMemcpyAsync(fr.jobs(), fr.host, jobs.size() * sizeof(Pbs1Job),
+                               hipMemcpyHostToDevice, st1),
+                "scheduler job table");
+        HIP_TRY(launch_blind_rotate1_jobs(fr.jobs(), fr.njobs, total, int(ctx->p.n0), ctx->bk1,
+                                          fr.cursor(), fr.done(), fr.stamps() + 1, st1),
+                "scheduler lvl1 bootstraps");
+    }
+    for (Req *r : reqs) HIP_TRY(hipEventRecord(r->done, st1), "hipEventRecord");
     return FHE_OK;
 }
 
 // serves every request not yet flushed, each kind in order on its stream
 int serve_pending(fhe_sched *s, hipStream_t st1, hipStream_t st2)
 {
-    auto stream_of = [&](int kind) { return kind == FHE_SCHED_PBS1 ? st1 : st2; };
-    std::vector<Req *> group;
+    std::vector<Req *> pbs1, group;
     int rows = 0;
     auto close_group = [&]() -> int {
         if (group.empty()) return FHE_OK;
-        const int rc = run_group(s, group, rows, stream_of(group[0]->kind));
+        const int rc = run_group(s, group, rows, st2);
         group.clear();
         rows = 0;
         return rc;
@@ -188,9 +233,13 @@ int serve_pending(fhe_sched *s, hipStream_t st1, hipStream_t st2)
     for (Req &r : s->reqs) {
         if (r.flushed) continue;
         r.flushed = true;
+        if (r.kind == FHE_SCHED_PBS1) {
+            pbs1.push_back(&r);
+            continue;
+        }
         if (r.B > s->max_rows) {
             SCHED_TRY(close_group());
-            SCHED_TRY(run_alone(s, r, stream_of(r.kind)));
+            SCHED_TRY(run_alone(s, r, st2));
             continue;
         }
         if (!group.empty() && (!same_group(*group[0], r) || rows + r.B > s->max_rows))
@@ -198,6 +247,7 @@ int serve_pending(fhe_sched *s, hipStream_t st1, hipStream_t st2)
         group.push_back(&r);
         rows += r.B;
     }
+    SCHED_TRY(serve_pbs1(s, pbs1, st1));
     return close_group();
 }
 
@@ -244,7 +294,9 @@ int fhe_sched_create(fhe_ctx *ctx, int max_rows, fhe_sched **out)
     auto *s = new fhe_sched;
     s->ctx = ctx;
     s->max_rows = max_rows;
-    hipError_t e = hipStreamCreateWithFlags(&s->s2, hipStreamNonBlocking);
+    hipError_t e = hipDeviceGetAttribute(&s->clock_khz, hipDeviceAttributeWallClockRate,
+                                         ctx->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->s2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->join, hipEventDisableTiming);
     if (e != hipSuccess) {
@@ -268,9 +320,13 @@ int fhe_sched_destroy(fhe_sched *s)
         if (r.start) (void)hipEventDestroy(r.start);
         if (r.done) (void)hipEventDestroy(r.done);
     }
-    void *bufs[] = {s->in1, s->tv1, s->out1, s->a2, s->b2, s->outc};
+    void *bufs[] = {s->a2, s->b2, s->outc};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    for (FlushRec &f : s->flushes) {
+        if (f.mem) (void)hipFree(f.mem);
+        if (f.host) (void)hipHostFree(f.host);
+    }
     (void)hipEventDestroy(s->fork);
     (void)hipEventDestroy(s->join);
     (void)hipStreamDestroy(s->s2);
@@ -354,7 +410,22 @@ int fhe_sched_wait(fhe_sched *s, uint64_t ticket, float *ms)
             return FHE_ERR_DEVICE;
         }
         HIP_TRY(hipEventSynchronize(r.done), "hipEventSynchronize");
-        if (ms) HIP_TRY(hipEventElapsedTime(ms, r.start, r.done), "hipEventElapsedTime");
+        if (ms && r.kind == FHE_SCHED_PBS1) {
+            // the flush's start and this request's completion, device clock
+            uint64_t t[2] = {0, 0};
+            const FlushRec &f = s->flushes[size_t(r.flush)];
+            HIP_TRY(hipMemcpy(&t[0], f.stamps(), sizeof(uint64_t), hipMemcpyDeviceToHost),
+                    "scheduler stamps");
+            if (r.job >= 0)
+                HIP_TRY(hipMemcpy(&t[1], f.stamps() + 1 + r.job, sizeof(uint64_t),
+                                  hipMemcpyDeviceToHost),
+                        "scheduler stamps");
+            else
+                t[1] = t[0];  // an empty request is done when its flush starts
+            *ms = float(double(t[1] - t[0]) / double(s->clock_khz));
+        } else if (ms) {
+            HIP_TRY(hipEventElapsedTime(ms, r.start, r.done), "hipEventElapsedTime");
+        }
         (void)hipEventDestroy(r.start);
         (void)hipEventDestroy(r.done);
         s->reqs.erase(s->reqs.begin() + long(i));

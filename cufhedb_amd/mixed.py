@@ -8,10 +8,13 @@ test/HE3_Query/TPCH_Q6.cu:179-193).  Two ways are measured here:
 * ``run``: every batch is one batched call on the context stream (a lvl1
   batch is one fhe_pbs_lvl01_dev launch, a lvl2 batch one fhe_compare_dev
   chain), timed with HIP events on its own;
-* ``run_coalesced``: the whole stream is submitted to the coalescing
-  scheduler (fhe_sched_*), which merges consecutive batches of one kind into
-  launches of up to ``max_rows`` rows; a batch's latency is the time from
-  the start of the flush to its completion.
+* ``run_coalesced``: the whole stream is submitted to the scheduler
+  (fhe_sched_*): every lvl1 batch runs in one persistent launch over a job
+  table (the waves take the next ciphertext of any batch, the key stays
+  resident, each batch's completion is stamped on the device clock), the
+  comparisons are merged into launches of up to ``max_rows`` rows on a
+  second stream; a batch's latency is the time from the start of the flush
+  to its completion.
 
 Results are decrypted and checked after the timed region in both.
 

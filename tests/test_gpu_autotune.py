@@ -114,3 +114,35 @@ def test_autotune_split_table(full636, knob):
     finally:
         ctx.set_tune_table([], knob=knob)
         ctx.set_tune_table([], knob=other)
+
+
+def test_tuning_cache_round_trip(full636, tmp_path):
+    """A tuned table saved to the per-device JSON cache comes back into a
+    fresh context of the same parameter set; another parameter set finds no
+    entry and keeps its built-in rule."""
+    k, ctx = full636
+    path = str(tmp_path / "tune.json")
+    tables = {F.TUNE_IKS21_TILE: [(1, 8), (512, 16), (2048, 32)],
+              F.TUNE_PBS1_WAVES: [(1, 2), (1024, 1)],
+              F.TUNE_IKS_SPLIT: [(1, 8), (256, 2)],
+              F.TUNE_IKS20_SPLIT: [(1, 4)]}
+    try:
+        for knob, t in tables.items():
+            ctx.set_tune_table(t, knob=knob)
+        ctx.save_tuning(path)
+    finally:
+        for knob in tables:
+            ctx.set_tune_table([], knob=knob)
+    fresh = F.Context(k.params)
+    try:
+        assert fresh.load_tuning(path)
+        for knob, t in tables.items():
+            assert fresh.tune_table(knob=knob) == t
+    finally:
+        fresh.close()
+    other = F.Context(F.preset(F.PRESET_REF672))
+    try:
+        assert not other.load_tuning(path)
+        assert other.tune_table(knob=F.TUNE_IKS21_TILE) == []
+    finally:
+        other.close()

@@ -49,11 +49,13 @@ def test_mixed_plan_shape():
 
 @pytest.mark.gpu
 def test_scheduler_matches_direct_calls(full672):
-    """Requests merged by the coalescing scheduler (fhe_sched_*) give the
+    """Requests served by the coalescing scheduler (fhe_sched_*) give the
     same ciphertexts, bit for bit, as one direct call per request: each
-    ciphertext's bootstrap is independent of its batch.  Covers merging of
-    consecutive same-kind requests, a split at a change of comparison,
-    requests larger than max_rows (served alone) and single-row requests."""
+    ciphertext's bootstrap is independent of its batch.  lvl1 requests all go
+    through one persistent launch over a job table (the one-wave blind
+    rotation, so the direct calls are made with one wave per ciphertext);
+    comparisons are merged, split at a change of comparison, served alone
+    above max_rows.  Single-row requests included."""
     k, ctx = full672
     rng = np.random.default_rng(77)
     spec = [("pbs1", 37, None), ("pbs1", 500, None), ("cmp2", 20, F.CMP_LE),
@@ -91,7 +93,12 @@ def test_scheduler_matches_direct_calls(full672):
         got = b.result()
         direct = F.DeviceBuffer(b.B * (F.LVL1_N + 1) * 4)
         if b.kind == "pbs1":
-            F.check(L.fhe_pbs_lvl01_dev(ctx.h, b.inp.ptr, b.aux.ptr, 0, direct.ptr, b.B, None))
+            try:
+                ctx.set_tune_table([(1, 1)], knob=F.TUNE_PBS1_WAVES)
+                F.check(L.fhe_pbs_lvl01_dev(ctx.h, b.inp.ptr, b.aux.ptr, 0, direct.ptr, b.B,
+                                            None))
+            finally:
+                ctx.set_tune_table([], knob=F.TUNE_PBS1_WAVES)
         else:
             F.check(L.fhe_compare_dev(ctx.h, 2, op, b.inp.ptr, b.aux.ptr, 1, direct.ptr, 20,
                                       F.LOGIC, b.B, None))
@@ -154,3 +161,24 @@ def test_summary_error_bounds():
     assert abs(out["cmp2"]["phase_err_std_log2"] - 20) < 1e-6
     assert out["cmp2"]["decision_margin_log2"] == 29.0
     assert out["errors"] == 0 and out["batches"] == 3
+
+
+@pytest.mark.gpu
+def test_mixed_stream_full_size_coalesced(full672):
+    """The mixed-parameter workload at its own size: 200 batches, sizes
+    lognormal around 2048 clipped to [256, 65536], 70 % lvl1 Z_8 LUT
+    bootstraps and 30 % lvl2 20-bit comparisons (HomMSB chain against one
+    encrypted threshold per batch), served by the coalescing scheduler in one
+    flush.  Every output decrypts to its plaintext result, and every output's
+    phase error stays more than a factor 2 inside its decision margin."""
+    k, ctx = full672
+    plan = mixed.make_plan(2026, 200)
+    sizes = [s for _, s in plan]
+    print(f"plan: {len(plan)} batches, {sum(sizes)} rows, largest {max(sizes)}", flush=True)
+    records, summary = mixed.run_coalesced(ctx, k.sk, k.params, plan)
+    print("mixed stream, coalesced:", summary)
+    assert len(records) == 200
+    assert summary["errors"] == 0
+    for kind in ("pbs1", "cmp2"):
+        e = summary[kind]
+        assert e["phase_err_max_log2"] < e["decision_margin_log2"] - 1, (kind, e)
