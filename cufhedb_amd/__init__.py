@@ -426,7 +426,7 @@ class Context:
                                        out_dev.ptr if out_dev is not None else None))
 
     # ---- autotuning (fhe_autotune / fhe_tune_get / fhe_tune_set)
-    TUNE_CANDIDATES = {TUNE_IKS21_TILE: (8, 16, 32), TUNE_PBS1_WAVES: (1, 2),
+    TUNE_CANDIDATES = {TUNE_IKS21_TILE: (8, 16, 32, 64), TUNE_PBS1_WAVES: (1, 2),
                        TUNE_IKS_SPLIT: (1, 2, 4, 8), TUNE_IKS20_SPLIT: (1, 2, 4, 8)}
 
     def autotune(self, batches=(64, 256, 1024, 4096, 16384), knob=None):

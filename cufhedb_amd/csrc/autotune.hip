@@ -31,7 +31,7 @@ struct Knob {
 bool knob_of(int knob, Knob &k)
 {
     switch (knob) {
-    case FHE_TUNE_IKS21_TILE: k = {3, {8, 16, 32, 0}}; return true;
+    case FHE_TUNE_IKS21_TILE: k = {4, {8, 16, 32, 64}}; return true;
     case FHE_TUNE_PBS1_WAVES: k = {2, {1, 2, 0, 0}}; return true;
     case FHE_TUNE_IKS_SPLIT:
     case FHE_TUNE_IKS20_SPLIT: k = {4, {1, 2, 4, 8}}; return true;

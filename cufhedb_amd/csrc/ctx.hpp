@@ -141,7 +141,7 @@ inline int tuned(const fhe_ctx *ctx, int knob, int B, int dflt)
 // 13.5 ms; kind 20 16384 ciphertexts 31.0 / 28.6 ms).
 inline int iks_tile(const fhe_ctx *ctx, int kind, int B)
 {
-    if (kind == 21) return tuned(ctx, FHE_TUNE_IKS21_TILE, B, B < 512 ? 8 : B < 2048 ? 16 : 32);
+    if (kind == 21) return tuned(ctx, FHE_TUNE_IKS21_TILE, B, B < 256 ? 8 : B < 512 ? 16 : 64);
     return tuned(ctx, kind == 10 ? FHE_TUNE_IKS_SPLIT : FHE_TUNE_IKS20_SPLIT, B, 8);
 }
 

@@ -196,6 +196,164 @@ __global__ __launch_bounds__(256) void k_iks_tiled(const TD *__restrict__ in,
     }
 }
 
+// Ciphertext-per-lane form for the lvl2 -> lvl1 switch (basebit 3, 10 digit
+// positions).  A wave switches 64 ciphertexts (one per lane) on CW output
+// columns; a workgroup's four waves take the same 64 ciphertexts on
+// adjacent column ranges (a slice of 4 CW columns, the body column TN
+// apart).  A mask word's 10 digit positions are taken in two stages of
+// H = 5: the stage's 35 key rows (7 per position, contiguous in the key)
+// are staged into LDS on the slice's columns, each position's rows behind a
+// zero row (slot 0, the row of a zero digit), double-buffered, the next
+// stage's rows in flight in registers while this stage's are used.  Each
+// lane turns its own digits into LDS row offsets (two VALU per position),
+// then reads its CW words of each selected row with 16-byte reads at
+// immediate offsets: the selection costs no scalar work and no per-read
+// address arithmetic.  Rows are padded by 16 bytes so that the (at most 8)
+// distinct rows a position's 64 lanes select fall on distinct banks.  The
+// waves of the first slice's wave 0 also switch the body column from a
+// 40-word LDS copy of the rows' column TN.
+template <class TD, int DN, int TN, int T, int BB, int CW>
+__global__ __launch_bounds__(256) void k_iks_ct(const TD *__restrict__ in,
+                                                const uint32_t *__restrict__ ksk,
+                                                uint32_t *__restrict__ out, int B)
+{
+    constexpr int W = int(sizeof(TD) * 8);
+    constexpr int ND = (1 << BB) - 1;  // candidate rows per digit position
+    constexpr int S = ND + 1;          // LDS rows per position (slot 0: zeros)
+    constexpr int H = 5;               // digit positions per stage
+    constexpr int NS = T / H;          // stages per mask word
+    constexpr int NG = DN * NS;        // stages
+    constexpr int SR = H * ND;         // key rows per stage
+    constexpr int SC = 4 * CW;         // slice columns
+    constexpr int RW = SC + 4;         // LDS row words (16 bytes of padding)
+    constexpr int EL = SR * SC;        // key words per stage and slice
+    constexpr int RPT = (EL + 255) / 256;
+    constexpr size_t row = size_t(TN) + 1;
+    static_assert(T % H == 0 && H == 5 && NS == 2, "two stages of five positions");
+    static_assert(CW % 8 == 0 && TN % SC == 0 && (BB * T) <= 32, "shape");
+    __shared__ __align__(16) uint32_t cand[2][H * S][RW];
+    __shared__ uint32_t colk[2][H * S];  // column TN of the stage's rows
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int c0 = blockIdx.y * SC;
+    const bool body_wave = blockIdx.y == 0 && wv == 0;
+    const int b = blockIdx.x * 64 + lane;  // this lane's ciphertext
+    const TD *arow = in + size_t(min(b, B - 1)) * (DN + 1);
+    const TD prec = TD(1) << (W - (1 + BB * T));
+    for (int e = tid; e < 2 * H * RW; e += 256) cand[e / (H * RW)][(e / RW) % H * S][e % RW] = 0u;
+    if (tid < 2 * H) colk[tid / H][(tid % H) * S] = 0u;
+    // stage g holds key rows g SR + q (q < SR); thread tid fetches the
+    // words e = tid + 256 m of the SR x SC block (row e / SC, column e % SC)
+    // through a buffer descriptor: a lane offset fixed for the kernel plus
+    // scalar stage and row offsets, no per-load address arithmetic (the key
+    // of this switch is 587,776,000 bytes, inside a descriptor's 32-bit range)
+    static_assert(256 % SC == 0, "a fetch round covers whole rows");
+    constexpr uint32_t key_bytes = uint32_t(DN) * T * ND * uint32_t(row) * 4u;
+    const __amdgpu_buffer_rsrc_t kd = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t *>(ksk), 0, int(key_bytes), 0x00020000);
+    const uint32_t vo = (uint32_t(tid / SC) * uint32_t(row) + uint32_t(c0 + tid % SC)) * 4u;
+    const uint32_t vob = (uint32_t(min(tid, SR - 1)) * uint32_t(row) + TN) * 4u;
+    auto fetch = [&](int g, uint32_t *v, uint32_t &vb) {
+        const uint32_t so = uint32_t(g) * SR * uint32_t(row) * 4u;
+#pragma unroll
+        for (int m = 0; m < RPT; m++) {
+            const int e = tid + 256 * m;
+            constexpr uint32_t step = uint32_t(256 / SC) * uint32_t(row) * 4u;
+            if (m + 1 < RPT || e < EL)
+                v[m] = __builtin_amdgcn_raw_buffer_load_b32(kd, vo, so + m * step, 0);
+        }
+        if (tid < SR) vb = __builtin_amdgcn_raw_buffer_load_b32(kd, vob, so, 0);
+    };
+    auto stage = [&](int buf, const uint32_t *v, uint32_t vb) {
+#pragma unroll
+        for (int m = 0; m < RPT; m++) {
+            const int e = tid + 256 * m;
+            const int q = e / SC;
+            if (m + 1 < RPT || e < EL) cand[buf][(q / ND) * S + q % ND + 1][e % SC] = v[m];
+        }
+        if (tid < SR) colk[buf][(tid / ND) * S + tid % ND + 1] = vb;
+    };
+    uint32_t acc[CW];
+#pragma unroll
+    for (int c = 0; c < CW; c++) acc[c] = 0;
+    uint32_t accb = 0;  // body column (first slice, wave 0)
+    auto consume = [&](uint32_t dig, int h, int buf) {
+        const uint32_t sd = h ? dig : dig >> (BB * H);  // this stage's digits, low H * BB bits
+        const uint32_t *base = &cand[buf][0][wv * CW];
+        uint32_t off[H];
+#pragma unroll
+        for (int j = 0; j < H; j++) {
+            const uint32_t d = (sd >> (BB * (H - 1 - j))) & uint32_t(ND);
+            off[j] = (uint32_t(j * S) + d) * RW;
+        }
+#pragma unroll
+        for (int c8 = 0; c8 < CW; c8 += 8) {
+            uint4 x[H][2];
+#pragma unroll
+            for (int j = 0; j < H; j++) {
+                const uint4 *p = reinterpret_cast<const uint4 *>(base + off[j] + c8);
+                x[j][0] = p[0];
+                x[j][1] = p[1];
+            }
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                acc[c8 + 4 * u + 0] -= x[0][u].x + x[1][u].x + x[2][u].x + x[3][u].x + x[4][u].x;
+                acc[c8 + 4 * u + 1] -= x[0][u].y + x[1][u].y + x[2][u].y + x[3][u].y + x[4][u].y;
+                acc[c8 + 4 * u + 2] -= x[0][u].z + x[1][u].z + x[2][u].z + x[3][u].z + x[4][u].z;
+                acc[c8 + 4 * u + 3] -= x[0][u].w + x[1][u].w + x[2][u].w + x[3][u].w + x[4][u].w;
+            }
+            // sums done here: at most one chunk's reads in flight
+#pragma unroll
+            for (int c = c8; c < c8 + 8; c++) asm volatile("" : "+v"(acc[c]));
+        }
+        if (body_wave) {
+#pragma unroll
+            for (int j = 0; j < H; j++) accb -= colk[buf][off[j] / RW];
+        }
+    };
+    // mask words are loaded in blocks of AB, a block ahead of their use
+    constexpr int AB = 4;
+    static_assert(DN % AB == 0, "blocking");
+    uint32_t v[RPT], vb = 0;
+    TD ac[AB], an[AB];
+#pragma unroll
+    for (int w = 0; w < AB; w++) ac[w] = arow[w];
+    fetch(0, v, vb);
+    stage(0, v, vb);
+    __syncthreads();
+    for (int w0 = 0; w0 < DN; w0 += AB) {
+        if (w0 + AB < DN) {
+#pragma unroll
+            for (int w = 0; w < AB; w++) an[w] = arow[w0 + AB + w];
+        }
+#pragma unroll
+        for (int u = 0; u < AB * NS; u++) {
+            const int g = w0 * NS + u, h = u & 1;
+            const uint32_t dig = uint32_t((ac[u / NS] + prec) >> (W - T * BB));
+            if (g + 1 < NG) fetch(g + 1, v, vb);
+            consume(dig, h, h);
+            // keep the staging (which waits for the key rows) after the
+            // subtractions, so the rows' latency hides behind them
+#pragma unroll
+            for (int c = 0; c < CW; c++) asm volatile("" : "+v"(acc[c]));
+            if (g + 1 < NG) stage(h ^ 1, v, vb);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int w = 0; w < AB; w++) ac[w] = an[w];
+    }
+    if (b < B) {
+        uint32_t *o = out + size_t(b) * row + c0 + wv * CW;
+#pragma unroll
+        for (int c = 0; c < CW; c++) o[c] = acc[c];
+        if (body_wave) {
+            const TD x = in[size_t(b) * (DN + 1) + DN];
+            const uint32_t body = W == 32 ? uint32_t(x) : uint32_t((x + (TD(1) << (W - 33))) >> (W - 32));
+            out[size_t(b) * row + TN] = body + accb;
+        }
+    }
+}
+
 __global__ void k_tlwe1_lincomb(const uint32_t *__restrict__ a, const uint32_t *__restrict__ b,
                                 int sa, int sb, uint32_t offset, uint32_t *__restrict__ out,
                                 int B)
@@ -369,6 +527,14 @@ void launch_iks21_tiled(const uint64_t *in, const uint32_t *ksk, uint32_t *out, 
                        dim3(256), 0, s, in, ksk, out, tn, B);
 }
 
+template <int CW>
+void launch_iks21_ct(const uint64_t *in, const uint32_t *ksk, uint32_t *out, int B, hipStream_t s)
+{
+    const dim3 grid(unsigned((B + 63) / 64), unsigned(Lvl1::N / (4 * CW)));
+    hipLaunchKernelGGL((k_iks_ct<uint64_t, Lvl2::N, Lvl1::N, KS21::t, KS21::basebit, CW>), grid,
+                       dim3(256), 0, s, in, ksk, out, B);
+}
+
 hipError_t launch_iks(int kind, const void *in, const uint32_t *ksk, uint32_t *out, int tn, int B,
                       hipStream_t s, int tile)
 {
@@ -387,13 +553,17 @@ hipError_t launch_iks(int kind, const void *in, const uint32_t *ksk, uint32_t *o
         break;
     }
     case 21: {
-        // `tile` ciphertexts per workgroup (8, 16 or 32; 0 = 32), all 10 digit
-        // rows of a mask word in flight.  Larger tiles read each key row for
-        // more ciphertexts, smaller ones give small batches more workgroups
-        // (64-ciphertext tiles measured 20-120 % slower: register and SGPR
-        // pressure lower the occupancy more than the halved key traffic gains)
+        // `tile` ciphertexts per workgroup: 8, 16 or 32 (0 = 32) in the
+        // column-per-lane form, all 10 digit rows of a mask word in flight
+        // (larger tiles read each key row for more ciphertexts, smaller ones
+        // give small batches more workgroups); 64 in the ciphertext-per-lane
+        // form (LDS-staged rows, 16-byte reads; 4-5x faster from 1024
+        // ciphertexts up, slower below 512, where a workgroup's 64 lanes are
+        // mostly idle).  The second form is built for the lvl1 output width.
         const auto *x = static_cast<const uint64_t *>(in);
-        if (tile == 8)
+        if (tile == 64 && tn == Lvl1::N)
+            launch_iks21_ct<16>(x, ksk, out, B, s);
+        else if (tile == 8)
             launch_iks21_tiled<8>(x, ksk, out, tn, B, s);
         else if (tile == 16)
             launch_iks21_tiled<16>(x, ksk, out, tn, B, s);

@@ -343,18 +343,20 @@ int fhe_digit_testvectors(int op, int digits, int digit_bits, uint64_t threshold
 
 /* ---- tile autotuning ---------------------------------------------------- */
 /* No reference counterpart (the reference launches one ciphertext per call).
- * FHE_TUNE_IKS21_TILE: ciphertexts per workgroup (8, 16 or 32) of the
- * lvl2 -> lvl1 key switch -- more read each key row once for more
- * ciphertexts, fewer give small batches more workgroups.  fhe_autotune
+ * FHE_TUNE_IKS21_TILE: ciphertexts per workgroup of the lvl2 -> lvl1 key
+ * switch: 8, 16 or 32 with one output column per lane (more read each key
+ * row once for more ciphertexts, fewer give small batches more workgroups),
+ * or 64 with one ciphertext per lane (key rows staged in LDS; the fastest
+ * form from about 512 ciphertexts up).  fhe_autotune
  * times every candidate on the context's KSK 21 at the n ascending batch
  * sizes and keeps the fastest per size (a batch then uses the entry of the
- * largest size <= it; untuned: 8 below 512 ciphertexts, 16 below 2048,
- * else 32); choice (n, may be NULL) receives the
- * picks, ms (n x 3 in candidate order 8, 16, 32, may be NULL) the mean
+ * largest size <= it; untuned: 8 below 256 ciphertexts, 16 below 512,
+ * else 64); choice (n, may be NULL) receives the picks, ms (n x the
+ * knob's candidate count, in candidate order, may be NULL) the mean
  * times.  All candidates compute identical results.  fhe_tune_get /
  * fhe_tune_set read and restore the table (e.g. from a per-device cache). */
 enum fhe_tune_knob {
-    FHE_TUNE_IKS21_TILE = 0, /* ciphertexts per workgroup: 8, 16, 32          */
+    FHE_TUNE_IKS21_TILE = 0, /* ciphertexts per workgroup: 8, 16, 32, 64      */
     FHE_TUNE_PBS1_WAVES = 1, /* waves per ciphertext of the FP64 lvl1 blind
                                 rotation: 1, or 2 (small batches: each
                                 ciphertext's CMUX on two SIMDs); untuned: 2

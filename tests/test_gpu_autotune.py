@@ -1,8 +1,10 @@
 """Tile autotuning of the lvl2 -> lvl1 key switch (fhe_autotune, fhe_tune_*).
 
-Every tile (8, 16, 32 ciphertexts per workgroup) must give the oracle's
-key switch word for word (IdentityKeySwitch<lvl21param>,
-TFHEpp/src/keyswitch.cpp:12-48) at batch sizes that leave partial tiles;
+Every tile (8, 16, 32 ciphertexts per workgroup with a column per lane, 64
+with a ciphertext per lane) must give the oracle's key switch word for word
+(IdentityKeySwitch<lvl21param>, TFHEpp/src/keyswitch.cpp:12-48) at batch
+sizes that leave partial tiles, and the two forms must agree word for word
+at 4,096 ciphertexts;
 autotuning must leave results unchanged and record a valid table; bad
 tables are refused.
 """
@@ -25,13 +27,13 @@ def ksk21(full636):
     return F.keygen_ksk(k.sk, 21, k.seed + 5)
 
 
-@pytest.mark.parametrize("tile", [8, 16, 32])
+@pytest.mark.parametrize("tile", [8, 16, 32, 64])
 def test_every_tile_matches_oracle(full636, ksk21, tile):
     k, ctx = full636
     rng = np.random.default_rng(tile)
     try:
         ctx.set_tune_table([(1, tile)])
-        for B in (1, 7, 33, 100):
+        for B in (1, 7, 33, 100, 130):
             x = rng.integers(0, 2 ** 63, (B, F.LVL2_N + 1), dtype=np.uint64) * np.uint64(2) \
                 + rng.integers(0, 2, (B, F.LVL2_N + 1), dtype=np.uint64)
             got = ctx.iks(21, x)
@@ -39,6 +41,27 @@ def test_every_tile_matches_oracle(full636, ksk21, tile):
                 assert np.array_equal(got[b], O.iks64(x[b], ksk21, 1024, 10, 3)), (B, b)
     finally:
         ctx.set_tune_table([])
+
+
+def test_forms_agree_at_full_size(full636, ksk21):
+    """4,096 lvl2 ciphertexts of random words: the ciphertext-per-lane form
+    (tile 64) equals the column-per-lane form (tile 32) word for word, and
+    three rows equal the oracle."""
+    _, ctx = full636
+    rng = np.random.default_rng(4096)
+    B = 4096
+    x = rng.integers(0, 2 ** 63, (B, F.LVL2_N + 1), dtype=np.uint64) * np.uint64(2) \
+        + rng.integers(0, 2, (B, F.LVL2_N + 1), dtype=np.uint64)
+    try:
+        ctx.set_tune_table([(1, 32)])
+        cols = ctx.iks(21, x)
+        ctx.set_tune_table([(1, 64)])
+        lanes = ctx.iks(21, x)
+    finally:
+        ctx.set_tune_table([])
+    assert np.array_equal(lanes, cols)
+    for b in (0, 2049, B - 1):
+        assert np.array_equal(lanes[b], O.iks64(x[b], ksk21, 1024, 10, 3)), b
 
 
 def test_autotune_keeps_results(full636):
@@ -52,7 +75,7 @@ def test_autotune_keeps_results(full636):
                                              for b, (t, ms) in res.items()})
         table = ctx.tune_table()
         assert [b for b, _ in table] == [32, 256, 2048]
-        assert all(t in (8, 16, 32) for _, t in table)
+        assert all(t in (8, 16, 32, 64) for _, t in table)
         assert all(all(v > 0 for v in ms.values()) for _, ms in res.values())
         assert np.array_equal(ctx.iks(21, x), before)
         assert np.array_equal(ctx.iks(21, x[:5]), before[:5])   # below the first size
