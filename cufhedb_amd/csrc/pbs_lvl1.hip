@@ -33,15 +33,6 @@ constexpr int kPolyCd = 512;  // complex points per polynomial
 // (fft_processor_spqlios.cpp:97-117) is folded into the key, exactly.
 constexpr double kKeyScale1 = 1.0 / 512;
 
-// (X^a - 1) * poly at coefficient n, a in [0, 2N)
-__device__ __forceinline__ uint32_t rot_minus_one(const uint32_t *poly, uint32_t a, int n)
-{
-    uint32_t idx = (uint32_t(n) - a) & (2u * N1 - 1);
-    uint32_t v = poly[idx & (N1 - 1)];
-    if (idx & N1) v = 0u - v;
-    return v - poly[n];
-}
-
 // Gadget digits of x (DecompositionPolynomial, trgsw.cpp:21-39):
 //   digit_d = ((x + offset) >> (32 - 6(d+1)) & 63) - 32.
 // prep1 adds the offset and flips the top bit of every 6-bit field, so each
@@ -57,26 +48,6 @@ __device__ __forceinline__ double digit1p(uint32_t u, int d)
 }
 __device__ __forceinline__ double digit1(uint32_t x, int d) { return digit1p(prep1(x), d); }
 
-// Digit planes: the l digit polynomials of one accumulator column, kept in
-// LDS between the column's rows as signed 6-bit values in bytes.  Lane L
-// owns 16 bytes of each plane, byte 2j for coefficient L + 64 j and byte
-// 2j + 1 for coefficient L + 64 j + 512 (the order of the transform's input
-// registers), so each lane writes and reads only its own slots.
-// pack four prepared words' digit-d fields (raw 6-bit, sign-extended on read)
-__device__ __forceinline__ uint32_t pack_digits4(const uint32_t *u, int d)
-{
-    const int sh = 32 - Lvl1::Bgbit * (d + 1);
-    uint32_t w = __builtin_amdgcn_ubfe(u[0], sh, Lvl1::Bgbit);
-    w |= __builtin_amdgcn_ubfe(u[1], sh, Lvl1::Bgbit) << 8;
-    w |= __builtin_amdgcn_ubfe(u[2], sh, Lvl1::Bgbit) << 16;
-    w |= __builtin_amdgcn_ubfe(u[3], sh, Lvl1::Bgbit) << 24;
-    return w;
-}
-__device__ __forceinline__ double plane_digit(uint32_t w, int byte)
-{
-    return double(int32_t(__builtin_amdgcn_sbfe(int32_t(w), 8 * byte, Lvl1::Bgbit)));
-}
-
 // load one TRGSW row (both output columns) of the device key layout
 __device__ __forceinline__ void load_row(const cd *row, int lane, cd b0[8], cd b1[8])
 {
@@ -87,12 +58,13 @@ __device__ __forceinline__ void load_row(const cd *row, int lane, cd b0[8], cd b
     }
 }
 
-// LDS of one blind rotation: the accumulator, the transform tile and the
-// digit planes (20 KB: two waves per SIMD)
-struct Br1Lds {
+// LDS of one blind rotation: the accumulator and the transform tile (17 KB).
+// 16-byte alignment lets the tile's double2 accesses be single 16-byte LDS
+// operations (with 8-byte alignment the compiler splits them, and the
+// kernel spills 26 registers and runs 2.5x slower).
+struct alignas(16) Br1Lds {
     uint32_t acc[2 * N1];
     cd tile[kFftLdsElems];
-    uint4 planes[Lvl1::l][64];
 };
 
 // The blind rotation of one lvl0 ciphertext c0 by the wave: acc = (0,
@@ -105,7 +77,6 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
 {
     uint32_t *acc = lds.acc;
     cd *tile = lds.tile;
-    auto &planes = lds.planes;
     // acc = (0, X^bbar * tv), bbar = 2N - (b >> 21)   (no rounding)
     const uint32_t bbar = 2u * N1 - (c0[n0] >> (32 - 1 - Lvl1::nbit));
 #pragma unroll
@@ -128,8 +99,13 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
         for (int q = 0; q < 8; q++) fd0[q] = fd1[q] = cd{0.0, 0.0};
 #pragma unroll 1
         for (int c = 0; c < 2; c++) {
-            // digits of (X^abar - 1) acc_c for the column's l rows: rotation
-            // and preparation once per column, then into the digit planes
+            // (X^abar - 1) acc_c at the lane's 16 coefficients, prepared for
+            // digit extraction, kept in registers for the column's l rows.
+            // With t = 4 (n - abar) mod 8192, (X^abar acc)[n] is the word at
+            // byte (t & 4092) of the column, negated when bit 12 of t is set:
+            // s = 0 or ~0 from that bit, and (v ^ s) + (K - s - p) is
+            // +-v - p + K with no select.
+            uint32_t u[16];
             {
                 const uint32_t *poly = acc + c * N1;
                 // the rotation's index arithmetic is redone per column (an
@@ -137,16 +113,18 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
                 // registers held across the rows)
                 uint32_t ac = abar;
                 asm volatile("" : "+v"(ac));
-                uint32_t u[16];
+                const uint32_t b4 = 4u * (uint32_t(lane) - ac);
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
-                    u[2 * j] = prep1(rot_minus_one(poly, ac, lane + 64 * j));
-                    u[2 * j + 1] = prep1(rot_minus_one(poly, ac, lane + 64 * j + 512));
-                }
 #pragma unroll
-                for (int d = 0; d < Lvl1::l; d++)
-                    planes[d][lane] = uint4{pack_digits4(u, d), pack_digits4(u + 4, d),
-                                            pack_digits4(u + 8, d), pack_digits4(u + 12, d)};
+                    for (int h = 0; h < 2; h++) {
+                        const uint32_t t = b4 + 256u * uint32_t(j + 8 * h);
+                        const uint32_t v = poly[(t & 4092u) >> 2];
+                        const uint32_t sg = uint32_t(__builtin_amdgcn_sbfe(int32_t(t), 12, 1));
+                        const uint32_t pn = poly[lane + 64 * j + 512 * h];
+                        u[2 * j + h] = ((v ^ sg) + ((kDecOff1 - sg) - pn)) ^ kDigitFlip1;
+                    }
+                }
             }
 #pragma unroll 1
             for (int d = 0; d < Lvl1::l; d++) {
@@ -155,13 +133,9 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
                 // overlaps the transform
                 cd b0[8], b1[8];
                 load_row(bki + r * 2 * kPolyCd, lane, b0, b1);
-                const uint4 w = planes[d][lane];
-                const uint32_t wd[4] = {w.x, w.y, w.z, w.w};
                 cd v[8];
 #pragma unroll
-                for (int j = 0; j < 8; j++)
-                    v[j] = cd{plane_digit(wd[j >> 1], 2 * (j & 1)),
-                              plane_digit(wd[j >> 1], 2 * (j & 1) + 1)};
+                for (int j = 0; j < 8; j++) v[j] = cd{digit1p(u[2 * j], d), digit1p(u[2 * j + 1], d)};
                 fft512_fwd(v, tile, lane, tw);
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
