@@ -69,19 +69,12 @@ struct MfmaTables {
     float2 t[64][3];          // [lane]: T(n1, 2h), w^{4 n1}, w^{16 n1}  twiddle bases
 };
 
-__device__ __forceinline__ uint32_t rot_minus_one_m(const uint32_t *poly, uint32_t a, int n)
+// signed gadget digit d (DecompositionPolynomial, trgsw.cpp:21-39) of a word
+// prepared as (x + offset) ^ flip, the top bit of every 6-bit field flipped:
+// the field read as a signed 6-bit number is the digit
+constexpr uint32_t kDigitFlip1m = (1u << 31) | (1u << 25) | (1u << 19);
+__device__ __forceinline__ int digit1mp(uint32_t u, int d)
 {
-    uint32_t idx = (uint32_t(n) - a) & (2u * N1 - 1);
-    uint32_t v = poly[idx & (N1 - 1)];
-    if (idx & N1) v = 0u - v;
-    return v - poly[n];
-}
-
-// signed gadget digit d (DecompositionPolynomial, trgsw.cpp:21-39)
-__device__ __forceinline__ int digit1m(uint32_t x, int d)
-{
-    constexpr uint32_t flip = (1u << 31) | (1u << 25) | (1u << 19);
-    const uint32_t u = (x + kDecOff1m) ^ flip;
     return int32_t(u << (Lvl1::Bgbit * d)) >> (32 - Lvl1::Bgbit);
 }
 
@@ -363,18 +356,36 @@ k_blind_rotate1_mfma(const uint32_t *__restrict__ in, int n0, const float4 *__re
         // element k = 4 t + q of lane l is frequency 4 (l >> 4) + q + 16 (l & 15)
         // + 256 t), kept in registers for the products with every key limb
         float2 zf[Lvl1::rows][8];
+        // (X^abar - 1) acc_c at the lane's 16 coefficients, prepared for
+        // digit extraction once per column (at its first row) and kept in
+        // registers: with t = 4 (n - abar) mod 8192 the rotated word is at
+        // byte t & 4092 of the column, negated when bit 12 of t is set
+        // (s = 0 or ~0), and (v ^ s) + (K - s - p) is +-v - p + K
+        uint32_t u[16];
 #pragma unroll
         for (int r = 0; r < Lvl1::rows; r++) {
             const int c = r / Lvl1::l, d = r - c * Lvl1::l;
-            const uint32_t *poly = acc + c * N1;
+            if (d == 0) {
+                const uint32_t *poly = acc + c * N1;
+                const uint32_t b4 = 4u * (uint32_t(lane) - abar);
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const uint32_t t = b4 + 256u * uint32_t(j + 8 * h);
+                        const uint32_t v = poly[(t & 4092u) >> 2];
+                        const uint32_t sg = uint32_t(__builtin_amdgcn_sbfe(int32_t(t), 12, 1));
+                        const uint32_t pn = poly[lane + 64 * j + 512 * h];
+                        u[2 * j + h] = ((v ^ sg) + ((kDecOff1m - sg) - pn)) ^ kDigitFlip1m;
+                    }
+                }
+            }
             // digits as stage-A B operands: step s holds j = 4s..4s+3 (re, im)
             half8 b[2];
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                const int n = lane + 64 * j;
-                b[j >> 2][2 * (j & 3)] = _Float16(digit1m(rot_minus_one_m(poly, abar, n), d));
-                b[j >> 2][2 * (j & 3) + 1] =
-                    _Float16(digit1m(rot_minus_one_m(poly, abar, n + 512), d));
+                b[j >> 2][2 * (j & 3)] = _Float16(digit1mp(u[2 * j], d));
+                b[j >> 2][2 * (j & 3) + 1] = _Float16(digit1mp(u[2 * j + 1], d));
             }
             f32x4 zr[2], zi[2];
             // the stage constants are re-read (L1 / L2) per transform rather
