@@ -226,8 +226,6 @@ __global__ __launch_bounds__(256) void k_iks_ct(const TD *__restrict__ in,
     constexpr int SR = H * ND;         // key rows per stage
     constexpr int SC = 4 * CW;         // slice columns
     constexpr int RW = SC + 4;         // LDS row words (16 bytes of padding)
-    constexpr int EL = SR * SC;        // key words per stage and slice
-    constexpr int RPT = (EL + 255) / 256;
     constexpr size_t row = size_t(TN) + 1;
     static_assert(T % H == 0 && H == 5 && NS == 2, "two stages of five positions");
     static_assert(CW % 8 == 0 && TN % SC == 0 && (BB * T) <= 32, "shape");
@@ -243,33 +241,43 @@ __global__ __launch_bounds__(256) void k_iks_ct(const TD *__restrict__ in,
     for (int e = tid; e < 2 * H * RW; e += 256) cand[e / (H * RW)][(e / RW) % H * S][e % RW] = 0u;
     if (tid < 2 * H) colk[tid / H][(tid % H) * S] = 0u;
     // stage g holds key rows g SR + q (q < SR); thread tid fetches the
-    // words e = tid + 256 m of the SR x SC block (row e / SC, column e % SC)
-    // through a buffer descriptor: a lane offset fixed for the kernel plus
-    // scalar stage and row offsets, no per-load address arithmetic (the key
-    // of this switch is 587,776,000 bytes, inside a descriptor's 32-bit range)
-    static_assert(256 % SC == 0, "a fetch round covers whole rows");
+    // 16-byte groups e = tid + 256 m of the SR x SC block (row e / SC4,
+    // columns 4 (e % SC4) ..+3) through a buffer descriptor: a lane offset
+    // fixed for the kernel plus scalar stage and row offsets, no per-load
+    // address arithmetic (the key of this switch is 587,776,000 bytes,
+    // inside a descriptor's 32-bit range; multi-dword buffer loads need only
+    // 4-byte alignment, the key rows are 4100 bytes apart)
+    constexpr int SC4 = SC / 4;
+    constexpr int EL4 = SR * SC4;  // 16-byte groups per stage and slice
+    constexpr int RPT = (EL4 + 255) / 256;
+    static_assert(256 % SC4 == 0, "a fetch round covers whole rows");
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     constexpr uint32_t key_bytes = uint32_t(DN) * T * ND * uint32_t(row) * 4u;
     const __amdgpu_buffer_rsrc_t kd = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<uint32_t *>(ksk), 0, int(key_bytes), 0x00020000);
-    const uint32_t vo = (uint32_t(tid / SC) * uint32_t(row) + uint32_t(c0 + tid % SC)) * 4u;
+    const uint32_t vo = (uint32_t(tid / SC4) * uint32_t(row) + uint32_t(c0 + 4 * (tid % SC4))) * 4u;
     const uint32_t vob = (uint32_t(min(tid, SR - 1)) * uint32_t(row) + TN) * 4u;
-    auto fetch = [&](int g, uint32_t *v, uint32_t &vb) {
+    auto fetch = [&](int g, u32x4 *v, uint32_t &vb) {
         const uint32_t so = uint32_t(g) * SR * uint32_t(row) * 4u;
 #pragma unroll
         for (int m = 0; m < RPT; m++) {
             const int e = tid + 256 * m;
-            constexpr uint32_t step = uint32_t(256 / SC) * uint32_t(row) * 4u;
-            if (m + 1 < RPT || e < EL)
-                v[m] = __builtin_amdgcn_raw_buffer_load_b32(kd, vo, so + m * step, 0);
+            constexpr uint32_t step = uint32_t(256 / SC4) * uint32_t(row) * 4u;
+            if (m + 1 < RPT || e < EL4)
+                v[m] = __builtin_amdgcn_raw_buffer_load_b128(kd, vo, so + m * step, 0);
         }
         if (tid < SR) vb = __builtin_amdgcn_raw_buffer_load_b32(kd, vob, so, 0);
     };
-    auto stage = [&](int buf, const uint32_t *v, uint32_t vb) {
+    auto stage = [&](int buf, const u32x4 *v, uint32_t vb) {
 #pragma unroll
         for (int m = 0; m < RPT; m++) {
             const int e = tid + 256 * m;
-            const int q = e / SC;
-            if (m + 1 < RPT || e < EL) cand[buf][(q / ND) * S + q % ND + 1][e % SC] = v[m];
+            const int q = e / SC4;
+            if (m + 1 < RPT || e < EL4) {
+                const u32x4 x = v[m];
+                *reinterpret_cast<uint4 *>(&cand[buf][(q / ND) * S + q % ND + 1][4 * (e % SC4)]) =
+                    uint4{x[0], x[1], x[2], x[3]};
+            }
         }
         if (tid < SR) colk[buf][(tid / ND) * S + tid % ND + 1] = vb;
     };
@@ -314,7 +322,8 @@ __global__ __launch_bounds__(256) void k_iks_ct(const TD *__restrict__ in,
     // mask words are loaded in blocks of AB, a block ahead of their use
     constexpr int AB = 4;
     static_assert(DN % AB == 0, "blocking");
-    uint32_t v[RPT], vb = 0;
+    u32x4 v[RPT];
+    uint32_t vb = 0;
     TD ac[AB], an[AB];
 #pragma unroll
     for (int w = 0; w < AB; w++) ac[w] = arow[w];
@@ -561,7 +570,13 @@ hipError_t launch_iks(int kind, const void *in, const uint32_t *ksk, uint32_t *o
         // ciphertexts up, slower below 512, where a workgroup's 64 lanes are
         // mostly idle).  The second form is built for the lvl1 output width.
         const auto *x = static_cast<const uint64_t *>(in);
-        if (tile == 64 && tn == Lvl1::N)
+        // the second form with 16 output columns per wave, or 32 above 4096
+        // ciphertexts (measured equal at 4096, 2 % faster at 8192 and 16 %
+        // at 16384, where the wider slice's halved key traffic pays for its
+        // third wave per SIMD instead of a fourth)
+        if (tile == 64 && tn == Lvl1::N && B > 4096)
+            launch_iks21_ct<32>(x, ksk, out, B, s);
+        else if (tile == 64 && tn == Lvl1::N)
             launch_iks21_ct<16>(x, ksk, out, B, s);
         else if (tile == 8)
             launch_iks21_tiled<8>(x, ksk, out, tn, B, s);

@@ -4,7 +4,7 @@ Every tile (8, 16, 32 ciphertexts per workgroup with a column per lane, 64
 with a ciphertext per lane) must give the oracle's key switch word for word
 (IdentityKeySwitch<lvl21param>, TFHEpp/src/keyswitch.cpp:12-48) at batch
 sizes that leave partial tiles, and the two forms must agree word for word
-at 4,096 ciphertexts;
+at 4,096 and 6,000 ciphertexts;
 autotuning must leave results unchanged and record a valid table; bad
 tables are refused.
 """
@@ -43,13 +43,14 @@ def test_every_tile_matches_oracle(full636, ksk21, tile):
         ctx.set_tune_table([])
 
 
-def test_forms_agree_at_full_size(full636, ksk21):
-    """4,096 lvl2 ciphertexts of random words: the ciphertext-per-lane form
-    (tile 64) equals the column-per-lane form (tile 32) word for word, and
-    three rows equal the oracle."""
+@pytest.mark.parametrize("B", [4096, 6000])
+def test_forms_agree_at_full_size(full636, ksk21, B):
+    """4,096 and 6,000 lvl2 ciphertexts of random words (16 and 32 output
+    columns per wave in the ciphertext-per-lane form; 6,000 leaves a partial
+    tile): the ciphertext-per-lane form (tile 64) equals the column-per-lane
+    form (tile 32) word for word, and three rows equal the oracle."""
     _, ctx = full636
-    rng = np.random.default_rng(4096)
-    B = 4096
+    rng = np.random.default_rng(B)
     x = rng.integers(0, 2 ** 63, (B, F.LVL2_N + 1), dtype=np.uint64) * np.uint64(2) \
         + rng.integers(0, 2, (B, F.LVL2_N + 1), dtype=np.uint64)
     try:
