@@ -110,6 +110,91 @@ def cpu_baseline(threads, seconds_budget=12.0):
     }
 
 
+def gate_and_iks_rates(F, ctx, sk, p, B, reps, threads, seconds_budget=4.0):
+    """The config-1 gate (HomNAND = lvl1 -> lvl0 key switch + PBS) and the
+    lvl1 -> lvl0 key switch alone, at n0 = 672: the device path at batch B
+    (HIP events) beside the compiled reference TFHEpp on `threads` host
+    threads over a bounded sample (HomNAND<lvl1param>, IdentityKeySwitch
+    <lvl10param>; gate.cpp:50-65, keyswitch.cpp:12-48), keys identical on both
+    sides (the device key switch key regenerated into a buffer from the
+    context's seed and copied to the host).  None when the compiled reference
+    is not built."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+
+    ref = O.ref()
+    if ref is None:
+        return None
+    L_ = F.lib()
+    rng = np.random.default_rng(29)
+    mu, minus_mu = np.uint32(1 << 29), np.uint32((1 << 32) - (1 << 29))
+    bits_a, bits_b = rng.integers(0, 2, B), rng.integers(0, 2, B)
+    enc = lambda seed, bits: F.tlwe_encrypt_u32(
+        seed, 0, sk.s1, np.where(bits == 1, mu, minus_mu).astype(np.uint32), 2.0 ** -25)
+    ca, cb = enc(41, bits_a), enc(42, bits_b)
+    da, db = F.DeviceBuffer.from_array(ca), F.DeviceBuffer.from_array(cb)
+    dout = F.DeviceBuffer(B * (N + 1) * 4)
+    d0 = F.DeviceBuffer(B * (p.n0 + 1) * 4)
+    gate_ms = timed_on(F, ctx, lambda: F.check(L_.fhe_gate_lvl1_dev(
+        ctx.h, F.GATE_NAND, da.ptr, db.ptr, dout.ptr, F.LOGIC, B, None)), reps)
+    ph = F.tlwe_phase_u32(dout.to_array(np.uint32, (B, N + 1)), sk.s1).view(np.int32)
+    gate_err = int(np.sum((ph > 0) != ((bits_a & bits_b) == 0)))
+    iks_ms = timed_on(F, ctx, lambda: F.check(L_.fhe_iks_dev(ctx.h, 10, da.ptr, d0.ptr, B, None)),
+                      reps)
+    # the same key switching key on the host (5.6 GB)
+    kbytes = N * 2 * 1023 * (p.n0 + 1) * 4
+    dk = F.DeviceBuffer(kbytes + 256)
+    ctx.keygen_ksk(10, sk, 20261206, out_dev=dk)
+    ctx.sync()
+    ksk = dk.to_array(np.uint32, ((kbytes + 256) // 4,))
+    for b in (da, db, dout, d0, dk):
+        b.free()
+    bkfft = O.bk_to_fft32(F.keygen_bk_lvl01(sk, 20261213))
+    n0 = p.n0
+
+    def cpu_rate(fn):
+        fn(0)
+        t = time.perf_counter()
+        fn(1)
+        per = time.perf_counter() - t
+        total = max(threads, int(seconds_budget / max(per, 1e-5)))
+        total = (total + threads - 1) // threads * threads
+        with cf.ThreadPoolExecutor(threads) as ex:
+            t = time.perf_counter()
+            done = sum(ex.map(fn, range(total)))
+            wall = time.perf_counter() - t
+        return done / wall, done, wall
+
+    def nand(i):
+        out = np.zeros(N + 1, np.uint32)
+        ref.ref_hom_nand_lvl1(O.P(out), O.P(np.ascontiguousarray(ca[i % B])),
+                              O.P(np.ascontiguousarray(cb[i % B])), O.P(bkfft), O.P(ksk))
+        return 1
+
+    def iks(i):
+        out = np.zeros(n0 + 1, np.uint32)
+        ref.ref_iks_lvl10(O.P(out), O.P(np.ascontiguousarray(ca[i % B])), O.P(ksk))
+        return 1
+
+    nand_rate, nand_n, nand_wall = cpu_rate(nand)
+    iks_rate, iks_n, iks_wall = cpu_rate(iks)
+    del ksk
+    out = {
+        "hom_nand": {"unit": "gates/s", "gpu": B / gate_ms * 1e3, "gpu_batch": B,
+                     "gpu_errors": gate_err, "cpu": nand_rate,
+                     "cpu_sample": f"{nand_n} HomNAND<lvl1param> in {nand_wall:.2f} s"},
+        "iks_lvl10": {"unit": "key switches/s", "gpu": B / iks_ms * 1e3, "gpu_batch": B,
+                      "cpu": iks_rate,
+                      "cpu_sample": f"{iks_n} IdentityKeySwitch<lvl10param> in {iks_wall:.2f} s"},
+        "cores": threads, "n0": int(n0),
+    }
+    print(f"bench: HomNAND n0={n0}: GPU {out['hom_nand']['gpu']:.0f} gates/s at batch {B} "
+          f"({gate_err} errors), CPU {nand_rate:.0f} gates/s on {threads} threads; lvl1 -> lvl0 "
+          f"key switch: GPU {out['iks_lvl10']['gpu']:.0f}/s, CPU {iks_rate:.0f}/s",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def operator_context(F, which=None):
     """A context with every key the operator layer uses, by default on the
     reference's own parameter set (n0 = 672, alpha0 = 2^-16).  The lvl2
@@ -582,9 +667,12 @@ def main():
     # like-for-like comparison with the CPU baseline (compiled TFHEpp, whose
     # n0 is fixed at 672)
     gpu672 = None
+    gates672 = None
     if not args.no_query or args.mixed or not args.no_cpu_baseline:
         octx, osk, op = operator_context(F)
         gpu672 = pbs_lvl01_rate(F, octx, osk, op, B, 5)
+        if not args.no_cpu_baseline:
+            gates672 = gate_and_iks_rates(F, octx, osk, op, B, 5, min(16, os.cpu_count() or 1))
     if not args.no_query or args.mixed:
         res["autotune"] = autotune(F, octx)
         if not args.no_query:
@@ -635,6 +723,8 @@ def main():
             # both at n0 = 672: GPU FP64 kernel vs compiled TFHEpp on the host
             res["cpu_baseline"]["gpu_same_params"] = dict(
                 gpu672, gpu_over_cpu=gpu672["pbs_per_s"] / res["cpu_baseline"]["value"])
+        if gates672 is not None:
+            res["cpu_baseline"]["gate_and_key_switch"] = gates672
     else:
         res["cpu_baseline"] = None
     print(json.dumps(res))
