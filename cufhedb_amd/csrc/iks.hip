@@ -209,9 +209,10 @@ __global__ __launch_bounds__(256) void k_iks_tiled(const TD *__restrict__ in,
 // then reads its CW words of each selected row with 16-byte reads at
 // immediate offsets: the selection costs no scalar work and no per-read
 // address arithmetic.  Rows are padded by 16 bytes so that the (at most 8)
-// distinct rows a position's 64 lanes select fall on distinct banks.  The
-// waves of the first slice's wave 0 also switch the body column from a
-// 40-word LDS copy of the rows' column TN.
+// distinct rows a position's 64 lanes select fall on distinct banks.  Wave
+// 0 of the first slice's workgroups also switches the body column from a
+// 40-word LDS copy of the rows' column TN.  The kernel is bound by its LDS
+// instruction rate (about 30 LDS instructions per wave and stage).
 template <class TD, int DN, int TN, int T, int BB, int CW>
 __global__ __launch_bounds__(256) void k_iks_ct(const TD *__restrict__ in,
                                                 const uint32_t *__restrict__ ksk,
