@@ -98,11 +98,12 @@ __device__ __forceinline__ void fft1024q_fwd(cd v[4], cd *xch, int lane, int wav
     // the exchange area overlaps the wave tiles: every wave has left its tile
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 4; k++) xch[k * 256 + n0] = v[k];
+    for (int k = 0; k < 4; k++) xch[k * kTileQ + n0] = v[k];
     __syncthreads();
+    // wave k's sequence sits at the start of wave k's own tile, so the wave
+    // may go on into its tile without waiting for the others' reads
 #pragma unroll
-    for (int j = 0; j < 4; j++) v[j] = xch[wave * 256 + lane + 64 * j];
-    __syncthreads();
+    for (int j = 0; j < 4; j++) v[j] = xch[wave * kTileQ + lane + 64 * j];
     // step C: radix-4 over the registers (stride 64), twiddle W256^{L m1}
     dft4<1>(v[0], v[1], v[2], v[3]);
     twiddle4<false>(v, tw.t256);
@@ -118,11 +119,11 @@ __device__ __forceinline__ void fft1024q_inv(cd v[4], cd *xch, int lane, int wav
     // all waves finished their tiles before the exchange overwrites them
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 4; j++) xch[wave * 256 + lane + 64 * j] = v[j];
+    for (int j = 0; j < 4; j++) xch[wave * kTileQ + lane + 64 * j] = v[j];
     __syncthreads();
     const int n0 = lane + 64 * wave;
 #pragma unroll
-    for (int k = 0; k < 4; k++) v[k] = xch[k * 256 + n0];
+    for (int k = 0; k < 4; k++) v[k] = xch[k * kTileQ + n0];
     __syncthreads();
     {
         cd g = tw.beta;
@@ -280,8 +281,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             acc[N2q + n] += trunc_torus64q(fd1[p].x);
             acc[N2q + n + 1024] += trunc_torus64q(fd1[p].y);
         }
-        __syncthreads();
+        // no barrier here: each lane updated only its own accumulator words,
+        // the inverse transform's last barrier follows every wave's reads of
+        // the exchange area, and the next forward transform waits for all
+        // waves before writing it
     }
+    __syncthreads();  // sample extraction reads other waves' words
 
     if (sample_extract) {
         uint64_t *o = out + size_t(ct) * (N2q + 1);
