@@ -58,14 +58,21 @@ __device__ __forceinline__ void load_row(const cd *row, int lane, cd b0[8], cd b
     }
 }
 
-// LDS of one blind rotation: the accumulator and the transform tile (17 KB).
-// 16-byte alignment lets the tile's double2 accesses be single 16-byte LDS
-// operations (with 8-byte alignment the compiler splits them, and the
-// kernel spills 26 registers and runs 2.5x slower).
+// LDS of one blind rotation: the accumulator's two columns with the
+// transform tile between them (17 KB).  Between transforms the tile's first
+// 16 rows of 64 words extend column 0 and its last 16 rows precede column 1
+// (see the rotation below).  16-byte alignment lets the tile's double2
+// accesses be single 16-byte LDS operations (with 8-byte alignment the
+// compiler splits them, and the kernel spills 26 registers and runs 2.5x
+// slower).
 struct alignas(16) Br1Lds {
-    uint32_t acc[2 * N1];
+    uint32_t acc0[N1];
     cd tile[kFftLdsElems];
+    uint32_t acc1[N1];
 };
+// words from column 0 to column 1
+constexpr int kCol1Words = N1 + kFftLdsElems * 4;
+static_assert(kFftLdsElems * 16 >= 2 * N1 * 4, "the tile holds both columns' extensions");
 
 // The blind rotation of one lvl0 ciphertext c0 by the wave: acc = (0,
 // X^bbar tv), n0 CMUX steps, then sample extraction (or the TRLWE) to out.
@@ -75,17 +82,17 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
                                                   uint32_t *__restrict__ out, int sample_extract,
                                                   Br1Lds &lds, int lane, const Twiddles1024 &tw)
 {
-    uint32_t *acc = lds.acc;
+    uint32_t *acc0 = lds.acc0, *acc1 = lds.acc1;
     cd *tile = lds.tile;
     // acc = (0, X^bbar * tv), bbar = 2N - (b >> 21)   (no rounding)
     const uint32_t bbar = 2u * N1 - (c0[n0] >> (32 - 1 - Lvl1::nbit));
 #pragma unroll
     for (int k = 0; k < N1 / 64; k++) {
         const int n = lane + 64 * k;
-        acc[n] = 0;
+        acc0[n] = 0;
         uint32_t idx = (uint32_t(n) - bbar) & (2u * N1 - 1);
         uint32_t v = tvp[idx & (N1 - 1)];
-        acc[N1 + n] = (idx & N1) ? 0u - v : v;
+        acc1[n] = (idx & N1) ? 0u - v : v;
     }
     __syncthreads();
 
@@ -99,32 +106,38 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
         for (int q = 0; q < 8; q++) fd0[q] = fd1[q] = cd{0.0, 0.0};
 #pragma unroll 1
         for (int c = 0; c < 2; c++) {
-            // (X^abar - 1) acc_c at the lane's 16 coefficients, prepared for
-            // digit extraction, kept in registers for the column's l rows.
-            // With t = 4 (n - abar) mod 8192, (X^abar acc)[n] is the word at
-            // byte (t & 4092) of the column, negated when bit 12 of t is set:
-            // s = 0 or ~0 from that bit, and (v ^ s) + (K - s - p) is
-            // +-v - p + K with no select.
+            // (X^abar - 1) acc_c at the lane's 16 coefficients n = lane + 64 k,
+            // prepared for digit extraction, kept in registers for the
+            // column's l rows.  With r = (lane - abar) mod 2048, (X^abar
+            // acc)[n] is acc_c[(r + 64 k) mod 1024], negated when (r + 64 k)
+            // mod 2048 >= 1024.  The column's negation is written as 16 rows
+            // of 64 words right after column 0 (the tile's first rows) or
+            // right before column 1 (its last rows); then word k sits at
+            // byte 4 (r mod 1024) + 256 k from one base, the wrap's sign
+            // included, and only the sign from bit 10 of r (s = 0 or ~0, the
+            // same for the 16 words; flipped for column 1, whose base is the
+            // negated rows) is left: (v ^ s) + (K - s - p) is +-v - p + K.
             uint32_t u[16];
             {
-                const uint32_t *poly = acc + c * N1;
-                // the rotation's index arithmetic is redone per column (an
-                // opaque copy of abar keeps it from being hoisted into
-                // registers held across the rows)
-                uint32_t ac = abar;
-                asm volatile("" : "+v"(ac));
-                const uint32_t b4 = 4u * (uint32_t(lane) - ac);
+                uint32_t *col = acc0 + c * kCol1Words;
+                uint32_t pn[16];
 #pragma unroll
-                for (int j = 0; j < 8; j++) {
+                for (int k = 0; k < 16; k++) pn[k] = col[lane + 64 * k];
+                uint32_t *neg = c == 0 ? col + N1 : col - N1;
 #pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        const uint32_t t = b4 + 256u * uint32_t(j + 8 * h);
-                        const uint32_t v = poly[(t & 4092u) >> 2];
-                        const uint32_t sg = uint32_t(__builtin_amdgcn_sbfe(int32_t(t), 12, 1));
-                        const uint32_t pn = poly[lane + 64 * j + 512 * h];
-                        u[2 * j + h] = ((v ^ sg) + ((kDecOff1 - sg) - pn)) ^ kDigitFlip1;
-                    }
-                }
+                for (int k = 0; k < 16; k++) neg[lane + 64 * k] = 0u - pn[k];
+                __syncthreads();
+                const uint32_t rr = uint32_t(lane) - abar;
+                const uint32_t *src = col - c * N1 + (rr & (N1 - 1));
+                const uint32_t s =
+                    uint32_t(__builtin_amdgcn_sbfe(int32_t(rr), 10, 1)) ^ (0u - uint32_t(c));
+                // K - s held opaque: three operations per word (a subtract,
+                // an xor-add, the flip); seen through, K - s - p is
+                // reassociated into four
+                uint32_t kc = kDecOff1 - s;
+                asm volatile("" : "+v"(kc));
+#pragma unroll
+                for (int k = 0; k < 16; k++) u[k] = ((src[64 * k] ^ s) + (kc - pn[k])) ^ kDigitFlip1;
             }
 #pragma unroll 1
             for (int d = 0; d < Lvl1::l; d++) {
@@ -135,7 +148,7 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
                 load_row(bki + r * 2 * kPolyCd, lane, b0, b1);
                 cd v[8];
 #pragma unroll
-                for (int j = 0; j < 8; j++) v[j] = cd{digit1p(u[2 * j], d), digit1p(u[2 * j + 1], d)};
+                for (int j = 0; j < 8; j++) v[j] = cd{digit1p(u[j], d), digit1p(u[j + 8], d)};
                 fft512_fwd(v, tile, lane, tw);
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
@@ -149,15 +162,15 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int n = lane + 64 * j;
-            acc[n] += trunc_torus32(fd0[j].x);
-            acc[n + 512] += trunc_torus32(fd0[j].y);
+            acc0[n] += trunc_torus32(fd0[j].x);
+            acc0[n + 512] += trunc_torus32(fd0[j].y);
         }
         fft512_inv(fd1, tile, lane, tw);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int n = lane + 64 * j;
-            acc[N1 + n] += trunc_torus32(fd1[j].x);
-            acc[N1 + n + 512] += trunc_torus32(fd1[j].y);
+            acc1[n] += trunc_torus32(fd1[j].x);
+            acc1[n + 512] += trunc_torus32(fd1[j].y);
         }
         __syncthreads();
     }
@@ -166,12 +179,15 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
 #pragma unroll
         for (int k = 0; k < N1 / 64; k++) {
             const int n = lane + 64 * k;
-            out[n] = n == 0 ? acc[0] : 0u - acc[N1 - n];
+            out[n] = n == 0 ? acc0[0] : 0u - acc0[N1 - n];
         }
-        if (lane == 0) out[N1] = acc[N1];
+        if (lane == 0) out[N1] = acc1[0];
     } else {
 #pragma unroll
-        for (int k = 0; k < 2 * N1 / 64; k++) out[lane + 64 * k] = acc[lane + 64 * k];
+        for (int k = 0; k < N1 / 64; k++) {
+            out[lane + 64 * k] = acc0[lane + 64 * k];
+            out[N1 + lane + 64 * k] = acc1[lane + 64 * k];
+        }
     }
 }
 
