@@ -146,14 +146,18 @@ __host__ __device__ constexpr int q_freq(int wave, int l, int q)
 }
 
 // trunc(x) mod 2^64 (execute_direct_torus64's truncation; |x| < 1 gives 0),
-// exact for |x| < 2^85: t = trunc(x) = H 2^32 + L, L in [0, 2^32), H mod 2^32
+// exact for |x| < 2^83: t = trunc(x) = H 2^32 + L with L in [0, 2^32) (exact
+// in FP64), and H mod 2^32 is the low mantissa word of H + 1.5 2^52 (exact
+// for |H| < 2^51).  The inverse transform's outputs are sums of 2 x 4 x 2048
+// products of a 9-bit digit and a 64-bit key word: RMS ~2^77, so 2^83 is
+// ~64 standard deviations away.
 __device__ __forceinline__ uint64_t trunc_torus64q(double x)
 {
     const double t = trunc(x);
     const double H = floor(t * 0x1p-32);
     const double L = __fma_rn(-H, 0x1p32, t);
-    const double Hm = __fma_rn(-floor(H * 0x1p-32), 0x1p32, H);
-    return (uint64_t(uint32_t(Hm)) << 32) | uint64_t(uint32_t(L));
+    const uint32_t Hm = uint32_t(__double_as_longlong(H + 0x1.8p52));
+    return (uint64_t(Hm) << 32) | uint64_t(uint32_t(L));
 }
 
 // x * i^k
@@ -181,6 +185,7 @@ __global__ __launch_bounds__(256) void k_rot_table2q(cd *__restrict__ rot)
 // read as a signed 9-bit number equals f - 256.
 constexpr uint64_t kDigitFlip2q = (1ull << 63) | (1ull << 54) | (1ull << 45) | (1ull << 36);
 __device__ __forceinline__ uint64_t prep2q(uint64_t x) { return (x + kDecOff2q) ^ kDigitFlip2q; }
+__device__ __forceinline__ uint64_t unprep2q(uint64_t u) { return (u ^ kDigitFlip2q) - kDecOff2q; }
 __device__ __forceinline__ double digit2qp(uint64_t u, int d)
 {
     const int sh = 64 - Lvl2::Bgbit * (d + 1);
@@ -213,12 +218,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const uint64_t *tvp = tv + size_t(tv_period > 0 ? ct % tv_period : ct) * tv_stride;
     const TwQ tw = make_twq(lane, wave);
 
+    // the accumulator is kept prepared for digit extraction: LDS holds
+    // prep2q(x) = (x + offset) ^ flip, so the l digit rows read their digits
+    // straight from it and only the update pays (one xor before and after
+    // the addition; the flip touches the high word only)
     const uint32_t bbar = 2u * N2q - (c0[n0] >> (32 - 1 - Lvl2::nbit));
     for (int n = tid; n < N2q; n += 256) {
-        acc[n] = 0;
+        acc[n] = prep2q(0);
         uint32_t idx = (uint32_t(n) - bbar) & (2u * N2q - 1);
         uint64_t v = tvp[idx & (N2q - 1)];
-        acc[N2q + n] = (idx & N2q) ? 0ull - v : v;
+        acc[N2q + n] = prep2q((idx & N2q) ? 0ull - v : v);
     }
     __syncthreads();
 
@@ -247,7 +256,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 #pragma unroll
             for (int p = 0; p < 4; p++) {  // digits of this lane's own accumulator words
                 const int n = lane + 64 * wave + 256 * p;
-                v[p] = cd{digit2qp(prep2q(poly[n]), d), digit2qp(prep2q(poly[n + 1024]), d)};
+                v[p] = cd{digit2qp(poly[n], d), digit2qp(poly[n + 1024], d)};
             }
             fft1024q_fwd<PERM>(v, xch, lane, wave, tw);
 #pragma unroll
@@ -271,15 +280,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             const int n = lane + 64 * wave + 256 * p;
-            acc[n] += trunc_torus64q(fd0[p].x);
-            acc[n + 1024] += trunc_torus64q(fd0[p].y);
+            acc[n] = prep2q(unprep2q(acc[n]) + trunc_torus64q(fd0[p].x));
+            acc[n + 1024] = prep2q(unprep2q(acc[n + 1024]) + trunc_torus64q(fd0[p].y));
         }
         fft1024q_inv<PERM>(fd1, xch, lane, wave, tw);
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             const int n = lane + 64 * wave + 256 * p;
-            acc[N2q + n] += trunc_torus64q(fd1[p].x);
-            acc[N2q + n + 1024] += trunc_torus64q(fd1[p].y);
+            acc[N2q + n] = prep2q(unprep2q(acc[N2q + n]) + trunc_torus64q(fd1[p].x));
+            acc[N2q + n + 1024] = prep2q(unprep2q(acc[N2q + n + 1024]) + trunc_torus64q(fd1[p].y));
         }
         // no barrier here: each lane updated only its own accumulator words,
         // the inverse transform's last barrier follows every wave's reads of
@@ -290,11 +299,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 
     if (sample_extract) {
         uint64_t *o = out + size_t(ct) * (N2q + 1);
-        for (int n = tid; n < N2q; n += 256) o[n] = n == 0 ? acc[0] : 0ull - acc[N2q - n];
-        if (tid == 0) o[N2q] = acc[N2q];
+        for (int n = tid; n < N2q; n += 256)
+            o[n] = n == 0 ? unprep2q(acc[0]) : 0ull - unprep2q(acc[N2q - n]);
+        if (tid == 0) o[N2q] = unprep2q(acc[N2q]);
     } else {
         uint64_t *o = out + size_t(ct) * 2 * N2q;
-        for (int n = tid; n < 2 * N2q; n += 256) o[n] = acc[n];
+        for (int n = tid; n < 2 * N2q; n += 256) o[n] = unprep2q(acc[n]);
     }
 }
 
