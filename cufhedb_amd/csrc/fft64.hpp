@@ -167,11 +167,31 @@ constexpr int kFftPad = 65;                  // row stride of transpose 2 (doubl
 constexpr int kFftPadA = 72;                 // row stride of transpose 1 (double2)
 constexpr int kFftLdsElems = 8 * kFftPadA;   // 576 double2 = 9216 bytes
 
+// Ordering around the transposes of a transform tile.  WAVE = true: the tile
+// belongs to this wave alone, and since LDS operations of one wave are
+// performed in order, a wavefront-scope fence suffices: it keeps the
+// compiler from moving LDS accesses across it and emits no wait, so a
+// transpose's reads queue right behind its writes instead of waiting for
+// them to complete.  WAVE = false: a workgroup barrier (the tile may be
+// read by another wave).
+template <bool WAVE>
+__device__ __forceinline__ void tile_sync()
+{
+    if (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // Forward transform: v[j] = z[L + 64 j] on entry, Z in permuted layout on exit.
 // TWIST = true: twisted negacyclic map above (w = exp(i pi/1024)).
 // TWIST = false: plain DFT of size 512, Z[m] = sum_n z_n exp(2 pi i n m/512)
 //                (tw.wl must then be 1).
-template <bool TWIST = true>
+// WAVE: the tile is private to the wave (see tile_sync).
+template <bool TWIST = true, bool WAVE = false>
 __device__ __forceinline__ void fft512_fwd(cd v[8], cd *lds, int lane, const Twiddles1024 &tw)
 {
     // twist constants, pass 1
@@ -191,13 +211,13 @@ __device__ __forceinline__ void fft512_fwd(cd v[8], cd *lds, int lane, const Twi
     // transpose 1: T[k1][L] -> lane (L0 + 8 k1) register L1
 #pragma unroll
     for (int k = 0; k < 8; k++) lds[k * kFftPadA + lane] = v[k];
-    __syncthreads();
+    tile_sync<WAVE>();
     {
         const int L0 = lane & 7, k1 = lane >> 3;
 #pragma unroll
         for (int l1 = 0; l1 < 8; l1++) v[l1] = lds[k1 * kFftPadA + L0 + 8 * l1];
     }
-    __syncthreads();
+    tile_sync<WAVE>();
     dft8<1>(v);
     {
         cd t = tw.w64;
@@ -211,18 +231,18 @@ __device__ __forceinline__ void fft512_fwd(cd v[8], cd *lds, int lane, const Twi
     // transpose 2: T[p][lane] -> lane (p + 8 k1) register L0
 #pragma unroll
     for (int k = 0; k < 8; k++) lds[k * kFftPad + lane] = v[k];
-    __syncthreads();
+    tile_sync<WAVE>();
     {
         const int p = lane & 7, k1 = lane >> 3;
 #pragma unroll
         for (int l0 = 0; l0 < 8; l0++) v[l0] = lds[p * kFftPad + l0 + 8 * k1];
     }
-    __syncthreads();
+    tile_sync<WAVE>();
     dft8<1>(v);
 }
 
 // Inverse transform (unnormalised: returns 512 * z): permuted Z in, natural z out.
-template <bool TWIST = true>
+template <bool TWIST = true, bool WAVE = false>
 __device__ __forceinline__ void fft512_inv(cd v[8], cd *lds, int lane, const Twiddles1024 &tw)
 {
     dft8<-1>(v);
@@ -231,10 +251,10 @@ __device__ __forceinline__ void fft512_inv(cd v[8], cd *lds, int lane, const Twi
 #pragma unroll
         for (int l0 = 0; l0 < 8; l0++) lds[p * kFftPad + l0 + 8 * k1] = v[l0];
     }
-    __syncthreads();
+    tile_sync<WAVE>();
 #pragma unroll
     for (int k = 0; k < 8; k++) v[k] = lds[k * kFftPad + lane];
-    __syncthreads();
+    tile_sync<WAVE>();
     {
         cd t = tw.w64;
         cd p = t;
@@ -250,10 +270,10 @@ __device__ __forceinline__ void fft512_inv(cd v[8], cd *lds, int lane, const Twi
 #pragma unroll
         for (int l1 = 0; l1 < 8; l1++) lds[k1 * kFftPadA + L0 + 8 * l1] = v[l1];
     }
-    __syncthreads();
+    tile_sync<WAVE>();
 #pragma unroll
     for (int k = 0; k < 8; k++) v[k] = lds[k * kFftPadA + lane];
-    __syncthreads();
+    tile_sync<WAVE>();
     {
         cd t = tw.wl;
 #pragma unroll

@@ -94,7 +94,7 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
         uint32_t v = tvp[idx & (N1 - 1)];
         acc1[n] = (idx & N1) ? 0u - v : v;
     }
-    __syncthreads();
+    tile_sync<true>();
 
     constexpr uint32_t kRound = 1u << (32 - 2 - Lvl1::nbit);
     for (int i = 0; i < n0; i++) {
@@ -126,7 +126,7 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
                 uint32_t *neg = c == 0 ? col + N1 : col - N1;
 #pragma unroll
                 for (int k = 0; k < 16; k++) neg[lane + 64 * k] = 0u - pn[k];
-                __syncthreads();
+                tile_sync<true>();
                 const uint32_t rr = uint32_t(lane) - abar;
                 const uint32_t *src = col - c * N1 + (rr & (N1 - 1));
                 const uint32_t s =
@@ -149,7 +149,7 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
                 cd v[8];
 #pragma unroll
                 for (int j = 0; j < 8; j++) v[j] = cd{digit1p(u[j], d), digit1p(u[j + 8], d)};
-                fft512_fwd(v, tile, lane, tw);
+                fft512_fwd<true, true>(v, tile, lane, tw);
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
                     cmac(fd0[q], v[q], b0[q]);
@@ -158,21 +158,21 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
             }
         }
         // every lane's rotation reads are complete (the FFTs synchronised)
-        fft512_inv(fd0, tile, lane, tw);
+        fft512_inv<true, true>(fd0, tile, lane, tw);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int n = lane + 64 * j;
             acc0[n] += trunc_torus32(fd0[j].x);
             acc0[n + 512] += trunc_torus32(fd0[j].y);
         }
-        fft512_inv(fd1, tile, lane, tw);
+        fft512_inv<true, true>(fd1, tile, lane, tw);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int n = lane + 64 * j;
             acc1[n] += trunc_torus32(fd1[j].x);
             acc1[n + 512] += trunc_torus32(fd1[j].y);
         }
-        __syncthreads();
+        tile_sync<true>();
     }
 
     if (sample_extract) {
