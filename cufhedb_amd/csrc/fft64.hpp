@@ -168,22 +168,15 @@ constexpr int kFftPadA = 72;                 // row stride of transpose 1 (doubl
 constexpr int kFftLdsElems = 8 * kFftPadA;   // 576 double2 = 9216 bytes
 
 // Ordering around the transposes of a transform tile.  WAVE = true: the tile
-// belongs to this wave alone, and since LDS operations of one wave are
-// performed in order, a wavefront-scope fence suffices: it keeps the
-// compiler from moving LDS accesses across it and emits no wait, so a
-// transpose's reads queue right behind its writes instead of waiting for
-// them to complete.  WAVE = false: a workgroup barrier (the tile may be
-// read by another wave).
+// belongs to this wave alone (wave_lds_sync: no wait, a transpose's reads
+// queue right behind its writes); WAVE = false: a workgroup barrier.
 template <bool WAVE>
 __device__ __forceinline__ void tile_sync()
 {
-    if (WAVE) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    } else {
+    if (WAVE)
+        wave_lds_sync();
+    else
         __syncthreads();
-    }
 }
 
 // Forward transform: v[j] = z[L + 64 j] on entry, Z in permuted layout on exit.
