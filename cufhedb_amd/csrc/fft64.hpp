@@ -282,6 +282,77 @@ __device__ __forceinline__ void fft512_inv(cd v[8], cd *lds, int lane, const Twi
     }
 }
 
+// Two inverse transforms (twisted, as fft512_inv) through one wave-private
+// tile, interleaved: each sequence's transpose is written while the other
+// one's arithmetic runs, and the second sequence's writes follow the first
+// one's reads of the same tile words with no wait (LDS operations of a wave
+// are performed in order), so the round trips overlap arithmetic of the
+// same wave.
+__device__ __forceinline__ void fft512_inv2(cd a[8], cd b[8], cd *lds, int lane,
+                                            const Twiddles1024 &tw)
+{
+    const int p = lane & 7, k1 = lane >> 3;
+    auto put2 = [&](const cd v[8]) {
+#pragma unroll
+        for (int l0 = 0; l0 < 8; l0++) lds[p * kFftPad + l0 + 8 * k1] = v[l0];
+    };
+    auto get2 = [&](cd v[8]) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = lds[k * kFftPad + lane];
+    };
+    auto put1 = [&](const cd v[8]) {
+#pragma unroll
+        for (int l1 = 0; l1 < 8; l1++) lds[k1 * kFftPadA + p + 8 * l1] = v[l1];
+    };
+    auto get1 = [&](cd v[8]) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = lds[k * kFftPadA + lane];
+    };
+    auto mid = [&](cd v[8]) {
+        cd t = tw.w64;
+        cd q = t;
+#pragma unroll
+        for (int k = 1; k < 8; k++) {
+            v[k] = cmulc(v[k], q);
+            if (k < 7) q = cmul(q, t);
+        }
+        dft8<-1>(v);
+    };
+    auto last = [&](cd v[8]) {
+        cd t = tw.wl;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            v[k] = cmulc(v[k], t);
+            if (k < 7) t = cmul(t, tw.w4l);
+        }
+        dft8<-1>(v);
+#pragma unroll
+        for (int j = 1; j < 8; j++) v[j] = cmulc(v[j], twist_const(j));
+    };
+    dft8<-1>(a);
+    put2(a);
+    dft8<-1>(b);
+    wave_lds_sync();
+    get2(a);
+    wave_lds_sync();
+    put2(b);
+    mid(a);
+    wave_lds_sync();
+    get2(b);
+    wave_lds_sync();
+    put1(a);
+    mid(b);
+    wave_lds_sync();
+    get1(a);
+    wave_lds_sync();
+    put1(b);
+    last(a);
+    wave_lds_sync();
+    get1(b);
+    wave_lds_sync();
+    last(b);
+}
+
 constexpr int kLaneTile = 320;  // double2 per wave tile of lanes64_fwd / lanes64_inv
 
 // v[q] *= base^q (CONJ: conj(base)^q), q = 1..3

@@ -158,14 +158,13 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
             }
         }
         // every lane's rotation reads are complete (the FFTs synchronised)
-        fft512_inv<true, true>(fd0, tile, lane, tw);
+        fft512_inv2(fd0, fd1, tile, lane, tw);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int n = lane + 64 * j;
             acc0[n] += trunc_torus32(fd0[j].x);
             acc0[n + 512] += trunc_torus32(fd0[j].y);
         }
-        fft512_inv<true, true>(fd1, tile, lane, tw);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int n = lane + 64 * j;
