@@ -21,13 +21,14 @@
 #include "fft64.hpp"
 #include "launch.hpp"
 #include "params.hpp"
+#include "rot1.hpp"
 
 namespace fhe {
 
 namespace {
 
 constexpr int N1 = Lvl1::N;
-constexpr uint32_t kDecOff1 = decomp_total_offset<uint32_t, 32, Lvl1::l, Lvl1::Bgbit>();
+constexpr uint32_t kDecOff1 = kBr1DecOff;
 constexpr int kPolyCd = 512;  // complex points per polynomial
 // Device-layout keys hold FD / 512: the inverse transform's 2/N scale
 // (fft_processor_spqlios.cpp:97-117) is folded into the key, exactly.
@@ -38,7 +39,7 @@ constexpr double kKeyScale1 = 1.0 / 512;
 // prep1 adds the offset and flips the top bit of every 6-bit field, so each
 // digit is then one signed 6-bit field extract: (f ^ 32) read as a signed
 // 6-bit number equals f - 32.
-constexpr uint32_t kDigitFlip1 = (1u << 31) | (1u << 25) | (1u << 19);
+constexpr uint32_t kDigitFlip1 = kBr1DigitFlip;
 __device__ __forceinline__ uint32_t prep1(uint32_t x) { return (x + kDecOff1) ^ kDigitFlip1; }
 __device__ __forceinline__ double digit1p(uint32_t u, int d)
 {
@@ -57,22 +58,6 @@ __device__ __forceinline__ void load_row(const cd *row, int lane, cd b0[8], cd b
         b1[q] = row[8 * 64 + q * 64 + lane];
     }
 }
-
-// LDS of one blind rotation: the accumulator's two columns with the
-// transform tile between them (17 KB).  Between transforms the tile's first
-// 16 rows of 64 words extend column 0 and its last 16 rows precede column 1
-// (see the rotation below).  16-byte alignment lets the tile's double2
-// accesses be single 16-byte LDS operations (with 8-byte alignment the
-// compiler splits them, and the kernel spills 26 registers and runs 2.5x
-// slower).
-struct alignas(16) Br1Lds {
-    uint32_t acc0[N1];
-    cd tile[kFftLdsElems];
-    uint32_t acc1[N1];
-};
-// words from column 0 to column 1
-constexpr int kCol1Words = N1 + kFftLdsElems * 4;
-static_assert(kFftLdsElems * 16 >= 2 * N1 * 4, "the tile holds both columns' extensions");
 
 // The blind rotation of one lvl0 ciphertext c0 by the wave: acc = (0,
 // X^bbar tv), n0 CMUX steps, then sample extraction (or the TRLWE) to out.
@@ -106,39 +91,10 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
         for (int q = 0; q < 8; q++) fd0[q] = fd1[q] = cd{0.0, 0.0};
 #pragma unroll 1
         for (int c = 0; c < 2; c++) {
-            // (X^abar - 1) acc_c at the lane's 16 coefficients n = lane + 64 k,
-            // prepared for digit extraction, kept in registers for the
-            // column's l rows.  With r = (lane - abar) mod 2048, (X^abar
-            // acc)[n] is acc_c[(r + 64 k) mod 1024], negated when (r + 64 k)
-            // mod 2048 >= 1024.  The column's negation is written as 16 rows
-            // of 64 words right after column 0 (the tile's first rows) or
-            // right before column 1 (its last rows); then word k sits at
-            // byte 4 (r mod 1024) + 256 k from one base, the wrap's sign
-            // included, and only the sign from bit 10 of r (s = 0 or ~0, the
-            // same for the 16 words; flipped for column 1, whose base is the
-            // negated rows) is left: (v ^ s) + (K - s - p) is +-v - p + K.
+            // the rotated column, prepared for digit extraction, in registers
+            // for the column's l rows
             uint32_t u[16];
-            {
-                uint32_t *col = acc0 + c * kCol1Words;
-                uint32_t pn[16];
-#pragma unroll
-                for (int k = 0; k < 16; k++) pn[k] = col[lane + 64 * k];
-                uint32_t *neg = c == 0 ? col + N1 : col - N1;
-#pragma unroll
-                for (int k = 0; k < 16; k++) neg[lane + 64 * k] = 0u - pn[k];
-                tile_sync<true>();
-                const uint32_t rr = uint32_t(lane) - abar;
-                const uint32_t *src = col - c * N1 + (rr & (N1 - 1));
-                const uint32_t s =
-                    uint32_t(__builtin_amdgcn_sbfe(int32_t(rr), 10, 1)) ^ (0u - uint32_t(c));
-                // K - s held opaque: three operations per word (a subtract,
-                // an xor-add, the flip); seen through, K - s - p is
-                // reassociated into four
-                uint32_t kc = kDecOff1 - s;
-                asm volatile("" : "+v"(kc));
-#pragma unroll
-                for (int k = 0; k < 16; k++) u[k] = ((src[64 * k] ^ s) + (kc - pn[k])) ^ kDigitFlip1;
-            }
+            br1_prep_column(lds, c, abar, lane, u);
 #pragma unroll 1
             for (int d = 0; d < Lvl1::l; d++) {
                 const int r = c * Lvl1::l + d;

@@ -49,6 +49,7 @@
 #include "fft64.hpp"
 #include "launch.hpp"
 #include "params.hpp"
+#include "rot1.hpp"
 
 namespace fhe {
 
@@ -59,7 +60,6 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int N1 = Lvl1::N;
-constexpr uint32_t kDecOff1m = decomp_total_offset<uint32_t, 32, Lvl1::l, Lvl1::Bgbit>();
 constexpr int kStrideB = 34;  // float2 row stride of the stage-A -> stage-B transpose
 
 // Per-lane constants of the two stages (built once on the device).
@@ -70,9 +70,8 @@ struct MfmaTables {
 };
 
 // signed gadget digit d (DecompositionPolynomial, trgsw.cpp:21-39) of a word
-// prepared as (x + offset) ^ flip, the top bit of every 6-bit field flipped:
-// the field read as a signed 6-bit number is the digit
-constexpr uint32_t kDigitFlip1m = (1u << 31) | (1u << 25) | (1u << 19);
+// prepared as (x + offset) ^ flip (br1_prep_column), the top bit of every
+// 6-bit field flipped: the field read as a signed 6-bit number is the digit
 __device__ __forceinline__ int digit1mp(uint32_t u, int d)
 {
     return int32_t(u << (Lvl1::Bgbit * d)) >> (32 - Lvl1::Bgbit);
@@ -271,7 +270,7 @@ __device__ __forceinline__ void mfma_fwd(const half8 b[2], const MfmaConsts &mc,
             if (tt < 3) tw = cmulf(tw, mc.t16);
         }
     }
-    __syncthreads();
+    wave_lds_sync();
     half8 yrh, yrl, yih, yil, nyih, nyil;
     {
         const float4 *row =
@@ -286,7 +285,7 @@ __device__ __forceinline__ void mfma_fwd(const half8 b[2], const MfmaConsts &mc,
             split2(x.w, hi, lo); yih[2 * p + 1] = hi; yil[2 * p + 1] = lo;
         }
     }
-    __syncthreads();
+    wave_lds_sync();
     nyih = -yih;
     nyil = -yil;
     // W32 columns 16..31 equal columns 0..15 times (-1)^{n1}: tile 1 runs on
@@ -323,8 +322,9 @@ k_blind_rotate1_mfma(const uint32_t *__restrict__ in, int n0, const float4 *__re
                      const MfmaTables *__restrict__ tab, const uint32_t *__restrict__ tv,
                      int tv_stride, uint32_t *__restrict__ out, int sample_extract, int B)
 {
-    __shared__ uint32_t acc[2 * N1];
-    __shared__ cd tile[kFftLdsElems];
+    __shared__ Br1Lds lds;  // one wave: all of it is the wave's own
+    uint32_t *acc0 = lds.acc0, *acc1 = lds.acc1;
+    cd *tile = lds.tile;
     const int lane = threadIdx.x;
     const int ct = blockIdx.x;
     if (ct >= B) return;
@@ -338,12 +338,12 @@ k_blind_rotate1_mfma(const uint32_t *__restrict__ in, int n0, const float4 *__re
 #pragma unroll
     for (int k = 0; k < N1 / 64; k++) {
         const int n = lane + 64 * k;
-        acc[n] = 0;
+        acc0[n] = 0;
         uint32_t idx = (uint32_t(n) - bbar) & (2u * N1 - 1);
         uint32_t v = tvp[idx & (N1 - 1)];
-        acc[N1 + n] = (idx & N1) ? 0u - v : v;
+        acc1[n] = (idx & N1) ? 0u - v : v;
     }
-    __syncthreads();
+    wave_lds_sync();
 
     // key slice of one (row, column, limb): 256 float4
     constexpr int kSlice = 256;
@@ -356,36 +356,20 @@ k_blind_rotate1_mfma(const uint32_t *__restrict__ in, int n0, const float4 *__re
         // element k = 4 t + q of lane l is frequency 4 (l >> 4) + q + 16 (l & 15)
         // + 256 t), kept in registers for the products with every key limb
         float2 zf[Lvl1::rows][8];
-        // (X^abar - 1) acc_c at the lane's 16 coefficients, prepared for
-        // digit extraction once per column (at its first row) and kept in
-        // registers: with t = 4 (n - abar) mod 8192 the rotated word is at
-        // byte t & 4092 of the column, negated when bit 12 of t is set
-        // (s = 0 or ~0), and (v ^ s) + (K - s - p) is +-v - p + K
+        // (X^abar - 1) acc_c at the lane's 16 coefficients (u[k]: n = lane +
+        // 64 k), prepared for digit extraction once per column (at its first
+        // row) and kept in registers (br1_prep_column)
         uint32_t u[16];
 #pragma unroll
         for (int r = 0; r < Lvl1::rows; r++) {
             const int c = r / Lvl1::l, d = r - c * Lvl1::l;
-            if (d == 0) {
-                const uint32_t *poly = acc + c * N1;
-                const uint32_t b4 = 4u * (uint32_t(lane) - abar);
-#pragma unroll
-                for (int j = 0; j < 8; j++) {
-#pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        const uint32_t t = b4 + 256u * uint32_t(j + 8 * h);
-                        const uint32_t v = poly[(t & 4092u) >> 2];
-                        const uint32_t sg = uint32_t(__builtin_amdgcn_sbfe(int32_t(t), 12, 1));
-                        const uint32_t pn = poly[lane + 64 * j + 512 * h];
-                        u[2 * j + h] = ((v ^ sg) + ((kDecOff1m - sg) - pn)) ^ kDigitFlip1m;
-                    }
-                }
-            }
+            if (d == 0) br1_prep_column(lds, c, abar, lane, u);
             // digits as stage-A B operands: step s holds j = 4s..4s+3 (re, im)
             half8 b[2];
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                b[j >> 2][2 * (j & 3)] = _Float16(digit1mp(u[2 * j], d));
-                b[j >> 2][2 * (j & 3) + 1] = _Float16(digit1mp(u[2 * j + 1], d));
+                b[j >> 2][2 * (j & 3)] = _Float16(digit1mp(u[j], d));
+                b[j >> 2][2 * (j & 3) + 1] = _Float16(digit1mp(u[j + 8], d));
             }
             f32x4 zr[2], zi[2];
             // the stage constants are re-read (L1 / L2) per transform rather
@@ -425,27 +409,28 @@ k_blind_rotate1_mfma(const uint32_t *__restrict__ in, int n0, const float4 *__re
                     const int m = 4 * (lane >> 4) + (k & 3) + 16 * (lane & 15) + 256 * (k >> 2);
                     buf[perm_slot(m)] = fd[k];
                 }
-                __syncthreads();
+                wave_lds_sync();
                 cd v[8];
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
                     const float2 f = buf[perm_slot(fd_index1024(lane, q))];
                     v[q] = cd{double(f.x), double(f.y)};
                 }
-                __syncthreads();
-                fft512_inv(v, tile, lane, tw64);
+                wave_lds_sync();
+                fft512_inv<true, true>(v, tile, lane, tw64);
                 // the digits of every row are already in zf, so acc may
                 // take each limb's share at once (lane-private words)
                 const int sh = limb_shift(lb);
+                uint32_t *col = acc0 + c * kBr1Col1Words;
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
-                    const int n = c * N1 + lane + 64 * j;
-                    acc[n] += round_torus32(v[j].x) << sh;
-                    acc[n + 512] += round_torus32(v[j].y) << sh;
+                    const int n = lane + 64 * j;
+                    col[n] += round_torus32(v[j].x) << sh;
+                    col[n + 512] += round_torus32(v[j].y) << sh;
                 }
             }
         }
-        __syncthreads();
+        wave_lds_sync();
     }
 
     if (sample_extract) {
@@ -453,13 +438,16 @@ k_blind_rotate1_mfma(const uint32_t *__restrict__ in, int n0, const float4 *__re
 #pragma unroll
         for (int k = 0; k < N1 / 64; k++) {
             const int n = lane + 64 * k;
-            o[n] = n == 0 ? acc[0] : 0u - acc[N1 - n];
+            o[n] = n == 0 ? acc0[0] : 0u - acc0[N1 - n];
         }
-        if (lane == 0) o[N1] = acc[N1];
+        if (lane == 0) o[N1] = acc1[0];
     } else {
         uint32_t *o = out + size_t(ct) * 2 * N1;
 #pragma unroll
-        for (int k = 0; k < 2 * N1 / 64; k++) o[lane + 64 * k] = acc[lane + 64 * k];
+        for (int k = 0; k < N1 / 64; k++) {
+            o[lane + 64 * k] = acc0[lane + 64 * k];
+            o[N1 + lane + 64 * k] = acc1[lane + 64 * k];
+        }
     }
 }
 

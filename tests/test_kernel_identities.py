@@ -2,19 +2,16 @@
 seeded random words on the CPU (no GPU): a wrong constant or index in the
 kernels' closed forms fails here before any device run.
 
-  * select-free rotation of the split-MFMA lvl1 blind rotation
-    (pbs_lvl1_mfma.hip): with t = 4 (n - a) mod 8192, the word of X^a p at
-    coefficient n is p[(t & 4092) / 4], negated when bit 12 of t is set, and
-    ((v ^ s) + (K - s - p_n)) ^ F, s = 0 or ~0 from that bit, equals the
-    prepared word ((X^a - 1) p)_n + K) ^ F of the plain rotation
-    (utils.hpp:110-141, trgsw.cpp:21-39);
-  * rotation from the extended columns of the FP64 lvl1 blind rotation
-    (pbs_lvl1.hip): LDS holds column 0, the 2304-word transform tile, then
+  * rotation from the extended columns of the lvl1 blind rotations
+    (rot1.hpp, used by pbs_lvl1.hip and pbs_lvl1_mfma.hip): LDS holds
+    column 0, the 2304-word transform tile, then
     column 1; column c's negation is written as 16 rows of 64 words after
     column 0 / before column 1, and with r = (L - a) mod 2^32 the lane's 16
     words are read at base + 64 k (base: column start, minus 1024 words for
     column 1, plus r mod 1024), s = 0 or ~0 from bit 10 of r, inverted for
-    column 1; ((v ^ s) + (K - s - p_n)) ^ F is again the prepared word;
+    column 1; ((v ^ s) + (K - s - p_n)) ^ F equals the prepared word
+    ((X^a - 1) p)_n + K) ^ F of the plain rotation (utils.hpp:110-141,
+    trgsw.cpp:21-39);
   * digit extraction from the prepared word: the signed 6-bit field d equals
     DecompositionPolynomial's digit d (trgsw.cpp:21-39).
 """
@@ -38,28 +35,9 @@ def rot_minus_one(p, a):
     return ((v - p.astype(np.int64)) % (1 << 32)).astype(np.uint64)
 
 
-def select_free(p, a):
-    """The kernels' form: t = 4 (n - a) mod 8192, s from bit 12 of t."""
-    n = np.arange(N, dtype=np.int64)
-    t = (4 * (n - a)) % 8192
-    v = p[(t & 4092) >> 2].astype(np.uint64)
-    s = np.where((t >> 12) & 1, np.uint64(M32), np.uint64(0))
-    q = (np.uint64(K) - s - p.astype(np.uint64)) & np.uint64(M32)
-    return (((v ^ s) + q) & np.uint64(M32)) ^ np.uint64(FLIP)
-
-
 def test_offset_closed_form():
     # offsetgen (trgsw.cpp:9-18) plus the rounding half: 2181562368 + 8192
     assert K == 2181562368 + 8192
-
-
-def test_select_free_rotation_every_shift():
-    rng = np.random.default_rng(2026)
-    p = rng.integers(0, 1 << 32, N, dtype=np.uint64).astype(np.uint32)
-    p[:4] = [0, 1, M32, 1 << 31]  # edge words
-    for a in range(2 * N):  # every rotation
-        want = ((rot_minus_one(p, a) + np.uint64(K)) & np.uint64(M32)) ^ np.uint64(FLIP)
-        assert np.array_equal(select_free(p, a), want), a
 
 
 def test_digit_fields_equal_decomposition():
@@ -82,22 +60,7 @@ def test_digit_fields_equal_decomposition():
     assert np.max(np.abs(gap)) <= 1 << (32 - L * BGBIT - 1)
 
 
-def test_planted_wrong_sign_bit_is_caught():
-    # taking the sign from bit 11 instead of bit 12 of t must fail the check
-    rng = np.random.default_rng(3)
-    p = rng.integers(0, 1 << 32, N, dtype=np.uint64).astype(np.uint32)
-    a = 700
-    n = np.arange(N, dtype=np.int64)
-    t = (4 * (n - a)) % 8192
-    v = p[(t & 4092) >> 2].astype(np.uint64)
-    s = np.where((t >> 11) & 1, np.uint64(M32), np.uint64(0))
-    q = (np.uint64(K) - s - p.astype(np.uint64)) & np.uint64(M32)
-    bad = (((v ^ s) + q) & np.uint64(M32)) ^ np.uint64(FLIP)
-    want = ((rot_minus_one(p, a) + np.uint64(K)) & np.uint64(M32)) ^ np.uint64(FLIP)
-    assert not np.array_equal(bad, want)
-
-
-# LDS words of the FP64 lvl1 blind rotation: column 0, tile (576 double2), column 1
+# LDS words of the lvl1 blind rotations: column 0, tile (576 double2), column 1
 TILE_WORDS = 576 * 4
 COL1 = N + TILE_WORDS
 
