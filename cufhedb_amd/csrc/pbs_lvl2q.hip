@@ -110,21 +110,26 @@ __device__ __forceinline__ void fft1024q_fwd(cd v[4], cd *xch, int lane, int wav
     lanes64_fwd<PERM>(v, xch + wave * kTileQ, lane, tw.t64, tw.t16);
 }
 
-template <bool PERM = false>
+// TAIL_SYNC: wait for every wave's exchange reads before returning (the
+// caller's next use of its own tile would overwrite words the others read);
+// not needed when the next write into the exchange area is a forward
+// transform's, which waits for all waves first.
+template <bool PERM = false, bool TAIL_SYNC = true>
 __device__ __forceinline__ void fft1024q_inv(cd v[4], cd *xch, int lane, int wave, const TwQ &tw)
 {
     lanes64_inv<PERM>(v, xch + wave * kTileQ, lane, tw.t64, tw.t16);
     twiddle4<true>(v, tw.t256);
     dft4<-1>(v[0], v[1], v[2], v[3]);
-    // all waves finished their tiles before the exchange overwrites them
-    __syncthreads();
+    // the wave writes its own tile, which no other wave reads or writes
+    // between the last barrier (a forward exchange's second, or the previous
+    // inverse's tail) and the reads below: no barrier before the write
 #pragma unroll
     for (int j = 0; j < 4; j++) xch[wave * kTileQ + lane + 64 * j] = v[j];
     __syncthreads();
     const int n0 = lane + 64 * wave;
 #pragma unroll
     for (int k = 0; k < 4; k++) v[k] = xch[k * kTileQ + n0];
-    __syncthreads();
+    if (TAIL_SYNC) __syncthreads();
     {
         cd g = tw.beta;
 #pragma unroll
@@ -283,7 +288,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             acc[n] = prep2q(unprep2q(acc[n]) + trunc_torus64q(fd0[p].x));
             acc[n + 1024] = prep2q(unprep2q(acc[n + 1024]) + trunc_torus64q(fd0[p].y));
         }
-        fft1024q_inv<PERM>(fd1, xch, lane, wave, tw);
+        // the next step's first forward exchange waits for all waves before
+        // writing into the others' tiles
+        fft1024q_inv<PERM, false>(fd1, xch, lane, wave, tw);
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             const int n = lane + 64 * wave + 256 * p;
@@ -291,9 +298,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             acc[N2q + n + 1024] = prep2q(unprep2q(acc[N2q + n + 1024]) + trunc_torus64q(fd1[p].y));
         }
         // no barrier here: each lane updated only its own accumulator words,
-        // the inverse transform's last barrier follows every wave's reads of
-        // the exchange area, and the next forward transform waits for all
-        // waves before writing it
+        // and the next forward transform's first barrier follows every wave's
+        // exchange reads above
     }
     __syncthreads();  // sample extraction reads other waves' words
 
