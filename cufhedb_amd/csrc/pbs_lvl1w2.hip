@@ -62,8 +62,10 @@ __device__ __forceinline__ cd c8w(int p)
     return p == 1 ? cd{c1, s1} : p == 2 ? cd{kR2, kR2} : cd{s1, c1};
 }
 
-// Forward transform.  xch: the workgroup's 2 x kLaneTile double2, holding
-// the exchange (512 double2) and aliasing the two wave tiles.
+// Forward transform.  xch: the workgroup's 2 x kLaneTile double2, the two
+// wave tiles; the exchange puts wave v's two sequences (k = 2v, 2v + 1) at
+// the start of v's own tile, so after the exchange a wave touches only its
+// own tile until the next exchange.
 template <bool PERM>
 __device__ __forceinline__ void fft512w_fwd(cd v[4], cd *xch, int lane, int wave, const TwW &tw)
 {
@@ -82,11 +84,10 @@ __device__ __forceinline__ void fft512w_fwd(cd v[4], cd *xch, int lane, int wave
     const int n0 = lane + 64 * wave;
     __syncthreads();  // both waves have left their tiles
 #pragma unroll
-    for (int k = 0; k < 4; k++) xch[k * 128 + n0] = v[k];
+    for (int k = 0; k < 4; k++) xch[(k >> 1) * kLaneTile + (k & 1) * 128 + n0] = v[k];
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 4; j++) v[j] = xch[(2 * wave + (j >> 1)) * 128 + lane + 64 * (j & 1)];
-    __syncthreads();  // the exchange is read before the tiles overwrite it
+    for (int j = 0; j < 4; j++) v[j] = xch[wave * kLaneTile + (j >> 1) * 128 + lane + 64 * (j & 1)];
 #pragma unroll
     for (int s = 0; s < 2; s++) {
         const cd a = v[2 * s], b = v[2 * s + 1];
@@ -97,7 +98,13 @@ __device__ __forceinline__ void fft512w_fwd(cd v[4], cd *xch, int lane, int wave
 }
 
 // Inverse (unnormalised: returns 512 z): frequency layout in, time layout out.
-template <bool PERM>
+// The wave writes its exchange words into its own tile, which the other wave
+// does not touch between the last barrier (the forward exchange's second, or
+// the previous inverse's tail) and the reads, so no barrier precedes the
+// write.  TAIL_SYNC: wait for both waves' exchange reads before returning
+// (the caller's next use of its tile would overwrite words the other wave
+// reads); not needed when a workgroup barrier follows anyway.
+template <bool PERM, bool TAIL_SYNC = true>
 __device__ __forceinline__ void fft512w_inv(cd v[4], cd *xch, int lane, int wave, const TwW &tw)
 {
     lanes64_inv<PERM>(v, xch + wave * kLaneTile, lane, tw.t64, tw.t16);
@@ -107,14 +114,13 @@ __device__ __forceinline__ void fft512w_inv(cd v[4], cd *xch, int lane, int wave
         v[2 * s] = cadd(a, b);
         v[2 * s + 1] = csub(a, b);
     }
-    __syncthreads();  // both waves have left their tiles
 #pragma unroll
-    for (int j = 0; j < 4; j++) xch[(2 * wave + (j >> 1)) * 128 + lane + 64 * (j & 1)] = v[j];
+    for (int j = 0; j < 4; j++) xch[wave * kLaneTile + (j >> 1) * 128 + lane + 64 * (j & 1)] = v[j];
     __syncthreads();
     const int n0 = lane + 64 * wave;
 #pragma unroll
-    for (int k = 0; k < 4; k++) v[k] = xch[k * 128 + n0];
-    __syncthreads();
+    for (int k = 0; k < 4; k++) v[k] = xch[(k >> 1) * kLaneTile + (k & 1) * 128 + n0];
+    if (TAIL_SYNC) __syncthreads();
     {
         cd g = tw.beta;
 #pragma unroll
@@ -225,7 +231,8 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             acc[0][nb + 128 * p] += trunc_torus32(fd0[p].x);
             acc[0][nb + 128 * p + 512] += trunc_torus32(fd0[p].y);
         }
-        fft512w_inv<PERM>(fd1, xch, lane, wave, tw);
+        // the step's closing barrier follows this transform's exchange reads
+        fft512w_inv<PERM, false>(fd1, xch, lane, wave, tw);
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             acc[1][nb + 128 * p] += trunc_torus32(fd1[p].x);
