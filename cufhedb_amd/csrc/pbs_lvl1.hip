@@ -1,9 +1,10 @@
 // pbs_lvl1.hip -- lvl1 (N = 1024, 32-bit torus) kernels for gfx950, FP64 path.
 //
 // One wave64 owns one ciphertext for the whole blind rotation: its
-// accumulator (2 x 1024 words) lives in LDS, the FFT exchanges go through a
-// private 8 KB LDS tile, and the frequency-domain accumulators of the
-// external product stay in registers.  A launch bootstraps a whole batch
+// accumulator (2 x 1024 words) lives in LDS with the wave's private 9 KB
+// transform tile between its two columns (rot1.hpp), ordered by wave-level
+// fences only, and the frequency-domain accumulators of the external
+// product stay in registers.  A launch bootstraps a whole batch
 // (one workgroup per ciphertext); every wave streams BK[i] (96 KB per CMUX)
 // from L2 / Infinity Cache, where the 62-66 MB key stays resident across
 // the batch.
