@@ -97,6 +97,12 @@ def cpu_baseline(threads, seconds_budget=12.0):
         t = time.perf_counter()
         done = sum(ex.map(one, range(total)))
         wall = time.perf_counter() - t
+    # one host thread over ~2 s of the same work (the per-core rate)
+    n1 = max(4, int(2.0 / max(per, 1e-4)))
+    t = time.perf_counter()
+    for i in range(n1):
+        one(i)
+    wall1 = time.perf_counter() - t
     impl = ("TFHEpp GateBootstrappingTLWE2TLWEFFT<lvl01param> (spqlios FP64 FFT) compiled from "
             "the reference sources, its fixed n0=672" if kind == "reference" else
             "oracle C restatement, n0=636")
@@ -107,6 +113,8 @@ def cpu_baseline(threads, seconds_budget=12.0):
         "kind": kind,
         "sample": f"{done} lvl1 PBS ({impl}), one ciphertext per thread at a time, "
                   f"{wall:.2f} s wall on {threads} host threads",
+        "single_thread": {"value": n1 / wall1, "unit": "PBS/s",
+                          "sample": f"{n1} lvl1 PBS on one host thread in {wall1:.2f} s"},
     }
 
 
