@@ -86,7 +86,9 @@ inline hipStream_t pick(fhe_ctx *ctx, void *stream)
 }
 
 // untuned: batches below this many ciphertexts use two waves per ciphertext
-constexpr int kPbs1TwoWaveBelow = 1024;
+// (one MI355X, n0 = 636: two waves 4.40 / 5.79 / 6.24 ms at 256 / 512 / 768
+// ciphertexts, one wave 5.17 / 5.41 / 5.94)
+constexpr int kPbs1TwoWaveBelow = 512;
 
 // set on this thread between fhe_graph_begin and fhe_graph_end: scratch
 // must not be (re)allocated while the context stream is being captured

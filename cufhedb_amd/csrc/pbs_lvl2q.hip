@@ -17,8 +17,7 @@
 //           a radix-4 pass over the registers, then the 64-point DFT over
 //           the lanes as three radix-4 passes: the first exchange by
 //           cross-lane register swaps (v_permlane32_swap /
-//           v_permlane16_swap) for batches up to kPermMaxBatch, through
-//           LDS above; the other two by wave-local LDS transposes
+//           v_permlane16_swap); the other two by wave-local LDS transposes
 //           (row strides 20 and 5 double2, conflict-free for gfx950's
 //           16-lanes-per-cycle b128 accesses).
 // Layouts: time domain wave w, lane L, register p holds z[L + 64 w + 256 p];
@@ -199,15 +198,12 @@ __device__ __forceinline__ double digit2qp(uint64_t u, int d)
 
 constexpr int kQRow = 4 * 4 * 64;  // double2 per (i, row, column): [wave][q][lane]
 
-// The first lane transpose as register swaps (PERM) trades LDS traffic for
-// VALU work: measured A/B on one box, 75.3 against 77.3 ms per 4096
-// bootstraps, but 388.4 against 376.5 ms per 20480 and 2.28 against 2.22 s
-// for the 2^14-row digit-wise comparison; on another, 148.0 against 146.9
-// ms per 8192 and 303.1 against 293.9 ms per 16384.  Batches up to this
-// size use it.
-constexpr int kPermMaxBatch = 4096;
-
-template <bool PERM>
+// The first lane transpose runs as register swaps (the transforms' PERM
+// form), which trades LDS traffic for VALU work: A/B in one call, 135.7
+// against 144.2 ms per 8192 bootstraps, 268.6 against 289.3 per 16384 and
+// 341.0 against 360.4 per 20480 (an earlier build of this kernel, with more
+// barriers and the accumulator decomposed per row, had found it faster only
+// up to 4096).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_blind_rotate2q(
     const uint32_t *__restrict__ in, int n0, const cd *__restrict__ bk,
     const cd *__restrict__ rot, const uint64_t *__restrict__ tv, int tv_stride, int tv_period,
@@ -263,7 +259,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 const int n = lane + 64 * wave + 256 * p;
                 v[p] = cd{digit2qp(poly[n], d), digit2qp(poly[n + 1024], d)};
             }
-            fft1024q_fwd<PERM>(v, xch, lane, wave, tw);
+            fft1024q_fwd<true>(v, xch, lane, wave, tw);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 cmac(fd0[q], v[q], b0[q]);
@@ -281,7 +277,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             fd1[q] = cmul(fd1[q], g);
         }
         // each lane reads and updates only its own accumulator words
-        fft1024q_inv<PERM>(fd0, xch, lane, wave, tw);
+        fft1024q_inv<true>(fd0, xch, lane, wave, tw);
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             const int n = lane + 64 * wave + 256 * p;
@@ -290,7 +286,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         }
         // the next step's first forward exchange waits for all waves before
         // writing into the others' tiles
-        fft1024q_inv<PERM, false>(fd1, xch, lane, wave, tw);
+        fft1024q_inv<true, false>(fd1, xch, lane, wave, tw);
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             const int n = lane + 64 * wave + 256 * p;
@@ -382,12 +378,8 @@ hipError_t launch_blind_rotate2q(const uint32_t *in, int n0, const cd *bk, const
                                  bool sample_extract, int B, hipStream_t s, int tv_period)
 {
     if (B <= 0) return hipSuccess;
-    if (B <= kPermMaxBatch)
-        hipLaunchKernelGGL(k_blind_rotate2q<true>, dim3(B), dim3(256), 0, s, in, n0, bk, rot, tv,
-                           tv_stride, tv_period, out, int(sample_extract), B);
-    else
-        hipLaunchKernelGGL(k_blind_rotate2q<false>, dim3(B), dim3(256), 0, s, in, n0, bk, rot,
-                           tv, tv_stride, tv_period, out, int(sample_extract), B);
+    hipLaunchKernelGGL(k_blind_rotate2q, dim3(B), dim3(256), 0, s, in, n0, bk, rot, tv, tv_stride,
+                       tv_period, out, int(sample_extract), B);
     return hipGetLastError();
 }
 

@@ -360,7 +360,7 @@ enum fhe_tune_knob {
     FHE_TUNE_PBS1_WAVES = 1, /* waves per ciphertext of the FP64 lvl1 blind
                                 rotation: 1, or 2 (small batches: each
                                 ciphertext's CMUX on two SIMDs); untuned: 2
-                                below 1024 ciphertexts */
+                                below 512 ciphertexts */
     FHE_TUNE_IKS_SPLIT = 2,  /* workgroups per ciphertext of the lvl1 -> lvl0
                                 key switch (kind 10; 1, 2, 4, 8: the digit
                                 positions split, sums added with integer

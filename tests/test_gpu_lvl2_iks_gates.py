@@ -112,11 +112,10 @@ def test_gpu_pbs_lvl02(lvl2, rng):
     ctx2.close()
 
 
-def test_gpu_pbs_lvl02_transpose_forms_agree(lvl2):
-    """Batches up to 4096 run the lvl2 transform's first lane transpose as
-    register swaps, larger ones through LDS: both move the same words, so
-    the 4096 common ciphertexts of a 4096- and a 4097-ciphertext batch give
-    the same outputs word for word."""
+def test_gpu_pbs_lvl02_batches_agree(lvl2):
+    """A bootstrap's output does not depend on the batch it runs in: the
+    4096 common ciphertexts of a 4096- and a 4097-ciphertext batch give the
+    same outputs word for word."""
     k, _, ctx = lvl2
     rng = np.random.default_rng(2008)
     B = 4097

@@ -6,7 +6,7 @@ FP64 with different operation orders, so the bar is the one of every
 bootstrap here: decrypt-exact, phase within 2^-6 of the oracle's, and equal
 noise statistics; the accumulator (no sample extraction) must sample-extract
 to the PBS output bit for bit.  The shape is chosen per batch size
-(FHE_TUNE_PBS1_WAVES; untuned: two waves below 1024 ciphertexts).
+(FHE_TUNE_PBS1_WAVES; untuned: two waves below 512 ciphertexts).
 """
 
 import numpy as np
