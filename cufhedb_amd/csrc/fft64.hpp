@@ -287,7 +287,10 @@ __device__ __forceinline__ void fft512_inv(cd v[8], cd *lds, int lane, const Twi
 // one's arithmetic runs, and the second sequence's writes follow the first
 // one's reads of the same tile words with no wait (LDS operations of a wave
 // are performed in order), so the round trips overlap arithmetic of the
-// same wave.
+// same wave.  The steps are fft512_inv's, written out again: built from
+// helper functions shared with fft512_inv instead, the same instructions
+// came out in another order and the blind rotation spilled 12 bytes per
+// lane.
 __device__ __forceinline__ void fft512_inv2(cd a[8], cd b[8], cd *lds, int lane,
                                             const Twiddles1024 &tw)
 {
