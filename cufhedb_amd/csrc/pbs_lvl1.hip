@@ -83,6 +83,11 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
     tile_sync<true>();
 
     constexpr uint32_t kRound = 1u << (32 - 2 - Lvl1::nbit);
+    // column 0 at the lane's 16 words, carried from one step's update to the
+    // next step's rotation (acc0 starts at zero)
+    uint32_t p0[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) p0[k] = 0;
     for (int i = 0; i < n0; i++) {
         const uint32_t abar = (c0[i] + kRound) >> (32 - 1 - Lvl1::nbit);
         if (abar == 0) continue;
@@ -90,12 +95,9 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
         cd fd0[8], fd1[8];
 #pragma unroll
         for (int q = 0; q < 8; q++) fd0[q] = fd1[q] = cd{0.0, 0.0};
-#pragma unroll 1
-        for (int c = 0; c < 2; c++) {
-            // the rotated column, prepared for digit extraction, in registers
-            // for the column's l rows
-            uint32_t u[16];
-            br1_prep_column(lds, c, abar, lane, u);
+        // a column's l rows, from the rotated column prepared for digit
+        // extraction (u, in registers for the column's rows)
+        auto rows = [&](int c, const uint32_t u[16]) {
 #pragma unroll 1
             for (int d = 0; d < Lvl1::l; d++) {
                 const int r = c * Lvl1::l + d;
@@ -113,14 +115,26 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
                     cmac(fd1[q], v[q], b1[q]);
                 }
             }
+        };
+        {
+            uint32_t u[16];
+            br1_prep_column_pn(lds, 0, abar, lane, p0, u);
+            rows(0, u);
+        }
+        {
+            uint32_t u[16];
+            br1_prep_column(lds, 1, abar, lane, u);
+            rows(1, u);
         }
         // every lane's rotation reads are complete (the FFTs synchronised)
         fft512_inv2(fd0, fd1, tile, lane, tw);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int n = lane + 64 * j;
-            acc0[n] += trunc_torus32(fd0[j].x);
-            acc0[n + 512] += trunc_torus32(fd0[j].y);
+            p0[j] = acc0[n] + trunc_torus32(fd0[j].x);
+            p0[j + 8] = acc0[n + 512] + trunc_torus32(fd0[j].y);
+            acc0[n] = p0[j];
+            acc0[n + 512] = p0[j + 8];
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) {

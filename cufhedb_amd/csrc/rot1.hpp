@@ -42,14 +42,13 @@ static_assert(kFftLdsElems * 16 >= 2 * Lvl1::N * 4, "the tile holds both columns
 // r (s = 0 or ~0, the same for the 16 words; flipped for column 1, whose base
 // is the negated rows) is left: (v ^ s) + (K - s - p) is +-v - p + K.  The
 // tile must be free (no transform in flight); the caller's wave owns lds.
-__device__ __forceinline__ void br1_prep_column(Br1Lds &lds, int c, uint32_t abar, int lane,
-                                                uint32_t u[16])
+// pn[k] is the column's word lane + 64 k, passed in by a caller that still
+// holds it from the accumulator update (br1_prep_column reads it from LDS).
+__device__ __forceinline__ void br1_prep_column_pn(Br1Lds &lds, int c, uint32_t abar, int lane,
+                                                   const uint32_t pn[16], uint32_t u[16])
 {
     constexpr int N1 = Lvl1::N;
     uint32_t *col = lds.acc0 + c * kBr1Col1Words;
-    uint32_t pn[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) pn[k] = col[lane + 64 * k];
     uint32_t *neg = c == 0 ? col + N1 : col - N1;
 #pragma unroll
     for (int k = 0; k < 16; k++) neg[lane + 64 * k] = 0u - pn[k];
@@ -63,6 +62,16 @@ __device__ __forceinline__ void br1_prep_column(Br1Lds &lds, int c, uint32_t aba
     asm volatile("" : "+v"(kc));
 #pragma unroll
     for (int k = 0; k < 16; k++) u[k] = ((src[64 * k] ^ s) + (kc - pn[k])) ^ kBr1DigitFlip;
+}
+
+__device__ __forceinline__ void br1_prep_column(Br1Lds &lds, int c, uint32_t abar, int lane,
+                                                uint32_t u[16])
+{
+    const uint32_t *col = lds.acc0 + c * kBr1Col1Words;
+    uint32_t pn[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) pn[k] = col[lane + 64 * k];
+    br1_prep_column_pn(lds, c, abar, lane, pn, u);
 }
 
 }  // namespace fhe
