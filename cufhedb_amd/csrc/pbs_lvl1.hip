@@ -84,7 +84,10 @@ __device__ __forceinline__ void blind_rotate1_one(const uint32_t *__restrict__ c
 
     constexpr uint32_t kRound = 1u << (32 - 2 - Lvl1::nbit);
     // column 0 at the lane's 16 words, carried from one step's update to the
-    // next step's rotation (acc0 starts at zero)
+    // next step's rotation (acc0 starts at zero); the two columns are written
+    // out below instead of looped over, so that these registers are dead
+    // during the rows' transforms (in a loop they would stay live through
+    // both columns and the kernel would spill)
     uint32_t p0[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) p0[k] = 0;
